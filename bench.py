@@ -1,0 +1,166 @@
+"""Benchmark of the MI355X walk kernel on the headline workload.
+
+Workload (the plain HIP kernel, single search on one MI355X): one seeded
+search (start scalar, Ed25519 key, 7-hex-digit prefix from workload seed 1),
+64 keys per batched inversion, one walk per lane over every CU, a budget of
+2^36 keys split evenly over the timed steps. A step is one launch of the walk
+kernel over every lane (inputs resident in HBM; launch and completion timed
+with HIP events on the kernel's own stream), and every hit is re-derived on
+the CPU before it counts.
+
+Prints one JSON line: keys/s, the kernel's roofline against the measured
+v_mad_u64_u32 issue peak, and the CPU oracle (the C restatement of the
+reference's CPU walk) timed on this host's cores on a bounded sample.
+
+    python bench.py [--steps K] [--warmup W] [--budget-log2 36] [--no-cpu]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lxmf-vanity_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import lxv  # noqa: E402
+
+METRIC = "keys/s on 1 MI355X vs plain HIP kernel; p50/p99 time-to-hit, mixed prefixes"
+
+
+def mad_work_per_key(batch):
+    """32x32->64 products per key on the 8x32-bit schoolbook count:
+    M(B) = 8 + 11/B multiplications and S(B) = 2 + 254/B squarings per key."""
+    return 64 * (8 + 11 / batch) + 36 * (2 + 254 / batch)
+
+
+def measure_mad_peak():
+    import ctypes
+    rate, clk, cus = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int(0)
+    L = lxv.lib()
+    L.lxv_debug_mad_rate.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+    if L.lxv_debug_mad_rate(0, ctypes.byref(rate), ctypes.byref(clk), ctypes.byref(cus)) != 0:
+        return None, None, None
+    return rate.value, clk.value, cus.value
+
+
+def cpu_baseline(seconds=12.0):
+    """The oracle walk (the reference's CPU path restated in C, one thread per
+    core as the reference runs, SHA-NI when present) on a bounded sample."""
+    import oracle_lib as orc
+    start, ed_seed, prefix = lxv.seeded_search(1, 7)
+    ed_pub = orc.ed25519_pub(ed_seed)
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    cores = min(cores, 16)
+    # calibrate with a short run, then size the sample to `seconds`
+    n = 1 << 18
+    t0 = time.perf_counter()
+    orc.scan_ranges(start, ed_pub, [(0, n)], prefix, threads=cores)
+    rate = n / (time.perf_counter() - t0)
+    n = max(1 << 20, int(rate * seconds))
+    t0 = time.perf_counter()
+    orc.scan_ranges(start, ed_pub, [(0, n)], prefix, threads=cores)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "keys/s", "cores": cores, "kind": "port",
+            "sample": "%d consecutive keys of the same seeded search (7-hex prefix), %.1f s, SHA-NI=%s"
+                      % (n, dt, orc.shani()),
+            "per_core": n / dt / cores}
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "walk_pmc.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--budget-log2", type=int, default=36)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--layout", type=int, default=lxv.LAYOUT_10X26)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build walks on a single MI355X and has no multi-GPU path "
+              "(a search shards by disjoint lane ranges with no exchange; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    start, ed_seed, prefix = lxv.seeded_search(1, 7)
+    ed_pub = lxv.ed25519_public(ed_seed)
+    search = lxv.GpuSearch(start=start, ed_seed=ed_seed, batch=args.batch, waves_per_cu=args.waves_per_cu,
+                           layout=args.layout)
+    b = search.backend
+    per_iter = b.threads * b.batch
+    budget = 1 << args.budget_log2
+    iters = max(1, budget // (per_iter * args.steps))
+    a = lxv.launch_args_for(ed_pub, prefix, iters=iters, max_hits=lxv.MAX_HITS)
+
+    for _ in range(args.warmup):
+        b.launch(a)
+
+    kernel_ms, keys, wall, hits = [], 0, 0.0, []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        h, _ = b.launch(a)
+        st = b.stats()
+        kernel_ms.append(st.kernel_ms)
+        keys += st.keys
+        hits += h
+    wall = time.perf_counter() - t0
+    # every hit re-derived from scratch on the CPU (src/gpu/mod.rs:157-160)
+    rechecked = 0
+    for t, n, addr in hits:
+        ident = search.identity(t, n)
+        if lxv.address_of_identity(ident) != addr or not addr.hex().startswith(prefix):
+            raise SystemExit("hit (lane %d, key %d) failed its CPU recheck" % (t, n))
+        rechecked += 1
+    gpu_s = sum(kernel_ms) / 1e3
+    value = keys / gpu_s
+
+    mad_peak, clock, cus = measure_mad_peak()
+    work = mad_work_per_key(b.batch)
+    achieved = value * work
+    roofline = {"bound": "valu-imad", "achieved": achieved / 1e12, "peak": (mad_peak or 0) / 1e12,
+                "unit": "T mad/s", "frac": (achieved / mad_peak) if mad_peak else None,
+                "traffic": None, "mads_per_key": work}
+    pmc = load_traffic()
+    if pmc:
+        roofline["traffic"] = pmc.get("hbm_bytes_per_launch")
+        roofline["traffic_note"] = pmc.get("note")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "keys/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": sum(kernel_ms) / len(kernel_ms), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32 (exact integer; i64 column sums)",
+        "data": "synthetic (seeded start scalar, Ed25519 key and prefix)",
+        "config": {"workload": "plain HIP walk, single search, 7-hex prefix, B=%d, 2^%d-key budget"
+                               % (b.batch, args.budget_log2),
+                   "lanes": b.threads, "keys_per_inversion": b.batch, "iters_per_step": iters,
+                   "prefix": prefix, "layout": "10x25.5" if args.layout == lxv.LAYOUT_10X26 else "8x32",
+                   "device": b.describe()},
+        "keys": keys, "wall_s": wall, "hits": len(hits), "hits_rechecked": rechecked,
+        "roofline": roofline,
+        "reference_gb10_keys_per_s": 630e6,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+    search.close()
+
+
+if __name__ == "__main__":
+    main()

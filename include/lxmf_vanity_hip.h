@@ -1,0 +1,197 @@
+/*
+ * lxmf_vanity_hip.h: C ABI of the MI355X (gfx950) backend for the LXMF vanity
+ * address search.
+ *
+ * This is the drop-in boundary for the reference's GPU search path. The
+ * reference drives its kernels through the Rust trait `Backend`
+ * (src/gpu/mod.rs:40-52), implemented for CUDA in src/gpu/cuda.rs:71-133,
+ * with the kernel ABI of src/gpu/kernel.cu:21-36. The entry points below map
+ * onto it one to one (each cites the interface it replaces); a Rust binding
+ * for them is given in INTEGRATION.md.
+ *
+ * Conventions (as the reference's backend, made explicit for a C ABI):
+ *  - Every call returns LXV_OK (0) or a negative LXV_E_* status; nothing
+ *    unwinds across the boundary. The message of the last failure on a
+ *    context is lxv_last_error(ctx).
+ *  - The caller owns every host buffer it passes; the library owns all
+ *    device memory and frees it in lxv_close.
+ *  - A context is bound to the thread that uses it, as the reference's
+ *    backend lives on one OS thread (src/main.rs:387). Calls on one context
+ *    must not overlap.
+ *  - Key numbering: lane t walks X25519 private keys start_t + 8 n for
+ *    n = 0, 1, 2, ...; a hit is reported as (t, n), exactly the pair
+ *    Backend::launch returns (src/gpu/common.h:349-355).
+ *  - All words are 32-bit: u-coordinates and q as little-endian words of the
+ *    32-byte encoding (src/gpu/mod.rs:69-71); the Ed25519 key, the name hash,
+ *    want/mask and addresses as big-endian words (src/gpu/mod.rs:74-84,
+ *    104-117).
+ */
+#ifndef LXMF_VANITY_HIP_H
+#define LXMF_VANITY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LXV_OK 0
+#define LXV_E_ARG -1        /* an argument is out of range */
+#define LXV_E_HIP -2        /* the HIP runtime failed (no GPU, no memory, ...) */
+#define LXV_E_STATE -3      /* the call is out of order (e.g. launch before init) */
+#define LXV_E_NOMEM -4      /* a host allocation failed */
+#define LXV_E_BUSY -5       /* a launch is still running (poll with wait = 0) */
+
+/* Field-element layouts the walk kernels are built for. */
+#define LXV_LAYOUT_8X32 0   /* 8 x 32-bit limbs, reduction modulo 2^256 - 38 */
+#define LXV_LAYOUT_10X26 1  /* ref10 10-limb signed radix 2^25.5 */
+
+typedef struct lxv_ctx lxv_ctx;
+
+/* Options for lxv_open. Zero-initialise, set `size`, override what you need. */
+typedef struct lxv_open_opts {
+    uint32_t size;         /* sizeof(lxv_open_opts) */
+    int32_t device;        /* HIP device ordinal; negative means 0 */
+    uint32_t threads;      /* walk lanes; 0 = CUs x waves_per_cu x 64 */
+    uint32_t batch;        /* keys per batched inversion (B); 0 = 64 */
+    uint32_t waves_per_cu; /* resident waves per CU the grid is sized for; 0 = 8 */
+    uint32_t layout;       /* LXV_LAYOUT_*; the 10x26 layout supports B <= 256 */
+} lxv_open_opts;
+
+/* What one launch needs besides the per-lane state: LaunchArgs,
+ * src/gpu/mod.rs:28-38, with the same fields and meanings. */
+typedef struct lxv_launch_args {
+    uint32_t iters;        /* batches of B keys per lane in this launch */
+    uint32_t ed[8];        /* Ed25519 public key, big-endian words */
+    uint32_t name_hash[3]; /* name hash: 10 bytes, last word holds 2 in its top half */
+    uint32_t want[4];      /* (address & mask) == want selects a hit */
+    uint32_t mask[4];
+    uint32_t q[8];         /* u(8B), little-endian words */
+    uint32_t max_hits;     /* hits recorded per launch (more are counted, not kept) */
+    uint32_t dump_threads; /* lanes that write their first batch's u to the dump */
+} lxv_launch_args;
+
+/* One hit: Backend::launch's (thread, n) pair plus the address the GPU
+ * computed for it, so the caller can compare without recomputing. */
+typedef struct lxv_hit {
+    uint32_t thread;
+    uint32_t reserved;
+    uint64_t n;
+    uint32_t address[4];   /* big-endian words of the 16-byte LXMF address */
+} lxv_hit;
+
+/* Timing of the last finished launch, measured with HIP events on the
+ * stream the walk kernel runs on. */
+typedef struct lxv_launch_stats {
+    double kernel_ms;      /* walk kernel time, launch to completion */
+    uint64_t keys;         /* keys walked by that launch */
+    uint32_t hits_total;   /* matches found (may exceed max_hits) */
+    uint32_t stopped;      /* 1 if lxv_stop ended it before iters batches */
+} lxv_launch_stats;
+
+/* Opens GPU `opts->device`, builds nothing at run time (kernels are compiled
+ * ahead of time for gfx950) and allocates the per-lane walk state.
+ * Replaces Cuda::new (src/gpu/cuda.rs:33-64) behind open_backend
+ * (src/gpu/mod.rs:60-67). */
+int lxv_open(const lxv_open_opts* opts, lxv_ctx** out);
+
+/* The GPU, for the start-up line. Backend::describe (src/gpu/mod.rs:43). */
+const char* lxv_describe(lxv_ctx* ctx);
+
+/* Backend::threads (src/gpu/mod.rs:44). */
+uint32_t lxv_threads(lxv_ctx* ctx);
+
+/* Keys per batched inversion. Backend::batch (src/gpu/mod.rs:46). */
+uint64_t lxv_batch(lxv_ctx* ctx);
+
+/* Sets up every lane from u(k-8) and u(k): 16 little-endian words per lane,
+ * u(k-8) first; n_words must be 16 x lxv_threads. Resets n and the
+ * checksum. Backend::init (src/gpu/mod.rs:48) and kernel init_walk
+ * (src/gpu/kernel.cu:21-24). */
+int lxv_init(lxv_ctx* ctx, const uint32_t* u_prev_cur, size_t n_words);
+
+/* Sets up every lane on the GPU itself: lane t starts at private key
+ * start + 8 (t x lane_stride), its u(k-8) and u(k) computed by an X25519
+ * ladder on the device. `start` must be a clamped scalar that
+ * random_start_scalar could return (src/main.rs:157-170) and
+ * lxv_threads x lane_stride must stay below 2^63. Replaces the host loop
+ * first_public_keys (src/gpu/mod.rs:210-228) for this layout of starts. */
+int lxv_init_seeded(lxv_ctx* ctx, const uint8_t start[32], uint64_t lane_stride);
+
+/* Enqueues one run of the walk kernel and returns without waiting: the
+ * first half of Backend::launch (src/gpu/mod.rs:51, src/gpu/cuda.rs:93-123).
+ * Fails with LXV_E_BUSY if the previous launch has not been polled. */
+int lxv_launch(lxv_ctx* ctx, const lxv_launch_args* args);
+
+/* Collects the result of the last launch: the second half of
+ * Backend::launch (src/gpu/cuda.rs:124-131). With wait = 0 it returns
+ * LXV_E_BUSY while the kernel runs; with wait != 0 it blocks. Fills up to
+ * `cap` hits (order unspecified, at most max_hits exist) and sets *n_hits;
+ * `dump` (dump_words >= dump_threads x B x 8, or NULL when dump_threads was
+ * 0) receives the first batch's u words of the dumped lanes, lane-major as
+ * the reference's dump (src/gpu/common.h:340-343). */
+int lxv_poll(lxv_ctx* ctx, int wait, lxv_hit* hits, uint32_t cap, uint32_t* n_hits,
+             uint32_t* dump, size_t dump_words);
+
+/* Asks a running launch to end early: every lane finishes the batch it is
+ * in, stores its state and exits, so the walk can resume with the next
+ * launch. No counterpart in the reference's trait; it is the "stop" of the
+ * host seam and what lets a long launch give way to a found hit. */
+int lxv_stop(lxv_ctx* ctx);
+
+/* XOR of the LXMF addresses of every key walked since the last init, as 4
+ * big-endian words (a parity checksum; the reference has no equivalent). */
+int lxv_checksum(lxv_ctx* ctx, uint32_t xor_words[4]);
+
+/* Per-lane key counters n, for lanes [first, first + count). */
+int lxv_read_ns(lxv_ctx* ctx, uint32_t first, uint32_t count, uint64_t* ns);
+
+/* Timing of the last polled launch. */
+int lxv_last_stats(lxv_ctx* ctx, lxv_launch_stats* out);
+
+/* Releases the context and all its device memory. Drop of Cuda. */
+void lxv_close(lxv_ctx* ctx);
+
+/* The message of the last failure on ctx (or of the last failed lxv_open
+ * when ctx is NULL); never NULL. The String side of Result<_, String>. */
+const char* lxv_last_error(lxv_ctx* ctx);
+
+/* ------------------------------------------------------------------------
+ * Host helpers of the search driver (the parts of GpuSearch and main.rs the
+ * backend's caller needs), so a caller of this C ABI needs no second crypto
+ * library. All re-derive from scratch on the CPU, independently of the GPU.
+ */
+
+/* X25519(k, 9) (RFC 7748); k is clamped. x25519-dalek as main.rs:194. */
+void lxv_x25519_base(const uint8_t k[32], uint8_t out[32]);
+/* Ed25519 public key of a 32-byte seed (RFC 8032); main.rs:195. */
+void lxv_ed25519_public(const uint8_t seed[32], uint8_t out[32]);
+/* The LXMF address of a 64-byte identity file (X25519 key || Ed25519 seed),
+ * computed the plain way: address_of_identity, src/main.rs:190-197. */
+void lxv_address_of_identity(const uint8_t identity[64], uint8_t out[16]);
+/* k + 8n as a little-endian 256-bit integer: step_scalar, src/main.rs:124-137. */
+void lxv_step_scalar(const uint8_t k[32], uint64_t n, uint8_t out[32]);
+/* 1 if k could come from random_start_scalar (src/main.rs:157-170). */
+int lxv_start_scalar_ok(const uint8_t k[32]);
+/* A random start scalar from the OS (random_start_scalar). Returns 0 or
+ * LXV_E_ARG if no system randomness is available. */
+int lxv_random_start_scalar(uint8_t out[32]);
+/* Parses 1-32 hex digits into want/mask words: Prefix::parse
+ * (src/main.rs:206-217) and prefix_words (src/gpu/mod.rs:74-84). Returns
+ * the number of digits, or LXV_E_ARG. */
+int lxv_prefix_words(const char* prefix, uint32_t want[4], uint32_t mask[4]);
+/* Fills the search-invariant words of a launch: ed (from ed_pub), the name
+ * hash of "lxmf.delivery" and q = u(8B), as GpuSearch::open does
+ * (src/gpu/mod.rs:102-117). */
+void lxv_launch_args_for(const uint8_t ed_pub[32], lxv_launch_args* args);
+/* The seeded workload generator used by the benchmarks and tests: a
+ * SHA-256 counter DRBG over (seed, label). Writes a start scalar that
+ * passes the rejection of src/main.rs:164-166, an Ed25519 seed, and the
+ * first `nibbles` hex digits of a prefix (prefix needs nibbles + 1 bytes). */
+void lxv_seeded_search(uint64_t seed, int nibbles, uint8_t start[32], uint8_t ed_seed[32], char* prefix);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

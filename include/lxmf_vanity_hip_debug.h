@@ -1,0 +1,48 @@
+/*
+ * lxmf_vanity_hip_debug.h: test entry points of the MI355X backend.
+ *
+ * They have no counterpart in the reference. They run the device field and
+ * hash code on given inputs, on the GPU or (the same source compiled for the
+ * host) on the CPU, so parity tests can check each piece of the walk against
+ * big-integer arithmetic and the oracle without going through a search.
+ */
+#ifndef LXMF_VANITY_HIP_DEBUG_H
+#define LXMF_VANITY_HIP_DEBUG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Field operation codes for lxv_debug_fe. */
+#define LXV_FE_MUL 0
+#define LXV_FE_SQ 1
+#define LXV_FE_ADD 2
+#define LXV_FE_SUB 3
+#define LXV_FE_INVERT 4
+#define LXV_FE_CANON 5
+#define LXV_FE_MUL121665 6
+
+/* out[i] = canonical encoding of op(a[i], b[i]) in limb layout `layout`
+ * (LXV_LAYOUT_*), for n elements of 8 little-endian words each (bit 255 of
+ * the inputs dropped). on_gpu selects the device kernel (device 0) or the
+ * host build of the same code. Returns LXV_OK or a negative status. */
+int lxv_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n, int on_gpu);
+
+/* out[i] = the LXMF address words (4 big-endian) of X25519 public key u[i]
+ * (8 little-endian words) with Ed25519 key ed and name hash nh, computed by
+ * the walk's specialised SHA-256 on the GPU or its host build. */
+int lxv_debug_address(const uint32_t ed[8], const uint32_t nh[3], const uint32_t* u, uint32_t* out, uint32_t n,
+                      int on_gpu);
+
+/* Measures the issue rate of v_mad_u64_u32 (32 x 32 -> 64 multiply-add) on
+ * `device` with a register-only probe kernel: lane-level mads per second, the
+ * peak the walk's multiply count is priced against. Also returns the
+ * device's clock and CU count. */
+int lxv_debug_mad_rate(int device, double* mads_per_s, double* clock_hz, int* cus);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,49 @@
+// Launchers of the device kernels (defined in lxv_kernels.hip) and the
+// field-operation dispatcher shared by the debug kernels and the host checks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lxv_sha.h"
+
+namespace lxv {
+
+struct WalkParams;
+
+enum DebugFeOp { kFeMul = 0, kFeSq = 1, kFeAdd = 2, kFeSub = 3, kFeInvert = 4, kFeCanon = 5, kFeMulSmall = 6 };
+
+// out = canonical words of op(a, b); a and b are decoded as 8 little-endian
+// words (bit 255 dropped). Sums and differences are multiplied by one before
+// encoding, as the walk only encodes multiplication results.
+template <class F>
+LXV_HD void debug_fe_op(int op, const uint32_t a[8], const uint32_t b[8], uint32_t out[8]) {
+    typedef typename F::fe fe;
+    const fe x = F::from_words(a), y = F::from_words(b);
+    fe r;
+    switch (op) {
+        case kFeMul: r = F::mul(x, y); break;
+        case kFeSq: r = F::sq(x); break;
+        case kFeAdd: r = F::mul(F::add(x, y), F::one()); break;
+        case kFeSub: r = F::mul(F::sub(x, y), F::one()); break;
+        case kFeInvert: r = F::invert(x); break;
+        case kFeMulSmall: r = F::mul_small(x, 121665); break;
+        default: r = F::mul(x, F::one()); break;
+    }
+    F::to_words(r, out);
+}
+
+bool walk_supported(int layout, int batch, int waves);
+int limbs_of(int layout);
+hipError_t launch_walk(int layout, int batch, int waves, const WalkParams& p, hipStream_t s);
+hipError_t launch_init_from_u(int layout, uint32_t* state, uint64_t* ns, uint32_t* xor_acc, const uint32_t* upc,
+                              uint32_t lanes, hipStream_t s);
+hipError_t launch_init_ladder(int layout, uint32_t* state, uint64_t* ns, uint32_t* xor_acc, const uint32_t base[8],
+                              const uint32_t q[8], uint64_t stride, uint32_t lanes, hipStream_t s);
+hipError_t launch_clear(uint32_t* words, uint32_t n, hipStream_t s);
+hipError_t launch_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n,
+                           hipStream_t s);
+// lanes must be a multiple of 256; 8 mads per lane per iteration.
+hipError_t launch_mad_rate(uint32_t* out, uint32_t lanes, uint32_t seed, uint32_t iters, hipStream_t s);
+hipError_t launch_debug_addr(const ShaSearch& c, const uint32_t* u, uint32_t* out, uint32_t n, hipStream_t s);
+
+}  // namespace lxv

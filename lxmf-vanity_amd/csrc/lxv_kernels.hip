@@ -1,0 +1,183 @@
+// HIP kernels for gfx950 and the host-side launchers the runtime calls.
+//
+//   walk           the per-lane walk (src/gpu/common.h:280-365 / kernel.cu:26-36),
+//                  one instantiation per (limb layout, keys per inversion,
+//                  waves per SIMD the registers are budgeted for)
+//   init_from_u    lane state from host-computed u(k-8), u(k)
+//                  (init_walk, src/gpu/kernel.cu:21-24)
+//   init_ladder    lane state from an on-device X25519 ladder
+//   clear          zero the per-launch counters (no hipMemsetAsync, so the
+//                  launch sequence can be captured in a graph)
+//   debug kernels  field operations and the address hash on given inputs, for
+//                  parity tests against the CPU
+#include "fe10.h"
+#include "lxv_kernels.h"
+#include "lxv_walk.h"
+
+namespace lxv {
+
+template <class F, int B, int W>
+__global__ __launch_bounds__(64, W) void walk_kernel(const WalkParams p) {
+    const uint32_t lane = blockIdx.x * 64u + threadIdx.x;
+    if (lane < p.lanes) walk_lane<F, B>(p, lane);
+}
+
+template <class F>
+__global__ __launch_bounds__(64) void init_from_u_kernel(uint32_t* state, uint64_t* ns, uint32_t* xor_acc,
+                                                         const uint32_t* upc, uint32_t lanes) {
+    const uint32_t lane = blockIdx.x * 64u + threadIdx.x;
+    if (lane < lanes) init_lane_from_u<F>(state, ns, xor_acc, upc, lanes, lane);
+}
+
+struct LadderArgs {
+    uint32_t base[8];  // start / 8 - 1
+    uint32_t q[8];
+    uint64_t stride;
+    uint32_t lanes;
+};
+
+template <class F>
+__global__ __launch_bounds__(64) void init_ladder_kernel(uint32_t* state, uint64_t* ns, uint32_t* xor_acc,
+                                                         const LadderArgs a) {
+    const uint32_t lane = blockIdx.x * 64u + threadIdx.x;
+    if (lane < a.lanes) init_lane_ladder<F>(state, ns, xor_acc, a.base, a.stride, a.q, a.lanes, lane);
+}
+
+__global__ void clear_kernel(uint32_t* words, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) words[i] = 0;
+}
+
+template <class F>
+__global__ void debug_fe_kernel(int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t wa[8], wb[8], wo[8];
+    for (int k = 0; k < 8; k++) {
+        wa[k] = a[i * 8 + k];
+        wb[k] = b[i * 8 + k];
+    }
+    debug_fe_op<F>(op, wa, wb, wo);
+    for (int k = 0; k < 8; k++) out[i * 8 + k] = wo[k];
+}
+
+__global__ void debug_addr_kernel(const ShaSearch c, const uint32_t* u, uint32_t* out, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t w[8], a[4];
+    for (int k = 0; k < 8; k++) w[k] = u[i * 8 + k];
+    lxmf_address_words(w, c, a);
+    for (int k = 0; k < 4; k++) out[i * 4 + k] = a[k];
+}
+
+// Issue-rate probe of v_mad_u64_u32 (the 32 x 32 -> 64 multiply-add the
+// field arithmetic is priced in): 8 independent accumulation chains per lane,
+// `iters` rounds. Reads and writes only out[lane].
+__global__ __launch_bounds__(256) void mad_rate_kernel(uint32_t* out, uint32_t seed, uint32_t iters) {
+    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t w0 = seed ^ lane, w1 = w0 * 3, w2 = w0 * 5, w3 = w0 * 7, w4 = w0 * 9, w5 = w0 * 11, w6 = w0 * 13,
+             w7 = w0 * 15;
+    const uint32_t a = seed + 1, b = seed * 7;
+    for (uint32_t it = 0; it < iters; it++) {
+        uint64_t c0, c1, c2, c3, c4, c5, c6, c7;
+        asm volatile(
+            "v_mad_u64_u32 %0, %8, %16, %17, %0\n v_mad_u64_u32 %1, %9, %16, %17, %1\n"
+            "v_mad_u64_u32 %2, %10, %16, %17, %2\n v_mad_u64_u32 %3, %11, %16, %17, %3\n"
+            "v_mad_u64_u32 %4, %12, %16, %17, %4\n v_mad_u64_u32 %5, %13, %16, %17, %5\n"
+            "v_mad_u64_u32 %6, %14, %16, %17, %6\n v_mad_u64_u32 %7, %15, %16, %17, %7"
+            : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5), "+v"(w6), "+v"(w7), "=s"(c0), "=s"(c1),
+              "=s"(c2), "=s"(c3), "=s"(c4), "=s"(c5), "=s"(c6), "=s"(c7)
+            : "v"(a), "v"(b));
+    }
+    out[lane] = (uint32_t)(w0 ^ w1 ^ w2 ^ w3 ^ w4 ^ w5 ^ w6 ^ (w7 >> 3));
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+
+typedef void (*walk_fn)(const WalkParams);
+
+#define LXV_WALK_ENTRY(F, LAYOUT, B, W) {LAYOUT, B, W, walk_kernel<F, B, W>}
+
+struct WalkEntry {
+    int layout;
+    int batch;
+    int waves;
+    walk_fn fn;
+};
+
+static const WalkEntry kWalks[] = {
+#define LXV_W3(F, LAYOUT, B) LXV_WALK_ENTRY(F, LAYOUT, B, 1), LXV_WALK_ENTRY(F, LAYOUT, B, 2), LXV_WALK_ENTRY(F, LAYOUT, B, 4)
+    LXV_W3(Field10, LXV_LAYOUT_10X26, 16),  LXV_W3(Field10, LXV_LAYOUT_10X26, 32),
+    LXV_W3(Field10, LXV_LAYOUT_10X26, 64),  LXV_W3(Field10, LXV_LAYOUT_10X26, 128),
+    LXV_W3(Field10, LXV_LAYOUT_10X26, 256),
+#undef LXV_W3
+};
+
+bool walk_supported(int layout, int batch, int waves) {
+    for (const auto& e : kWalks)
+        if (e.layout == layout && e.batch == batch && e.waves == waves) return true;
+    return false;
+}
+
+int limbs_of(int layout) { return Field10::kLimbs; (void)layout; }
+
+hipError_t launch_walk(int layout, int batch, int waves, const WalkParams& p, hipStream_t s) {
+    for (const auto& e : kWalks) {
+        if (e.layout == layout && e.batch == batch && e.waves == waves) {
+            const dim3 grid((p.lanes + 63) / 64), block(64);
+            hipLaunchKernelGGL(e.fn, grid, block, 0, s, p);
+            return hipGetLastError();
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_init_from_u(int layout, uint32_t* state, uint64_t* ns, uint32_t* xor_acc, const uint32_t* upc,
+                              uint32_t lanes, hipStream_t s) {
+    const dim3 grid((lanes + 63) / 64), block(64);
+    (void)layout;
+    hipLaunchKernelGGL(init_from_u_kernel<Field10>, grid, block, 0, s, state, ns, xor_acc, upc, lanes);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_ladder(int layout, uint32_t* state, uint64_t* ns, uint32_t* xor_acc, const uint32_t base[8],
+                              const uint32_t q[8], uint64_t stride, uint32_t lanes, hipStream_t s) {
+    LadderArgs a;
+    for (int i = 0; i < 8; i++) {
+        a.base[i] = base[i];
+        a.q[i] = q[i];
+    }
+    a.stride = stride;
+    a.lanes = lanes;
+    const dim3 grid((lanes + 63) / 64), block(64);
+    (void)layout;
+    hipLaunchKernelGGL(init_ladder_kernel<Field10>, grid, block, 0, s, state, ns, xor_acc, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear(uint32_t* words, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(clear_kernel, dim3((n + 255) / 256), dim3(256), 0, s, words, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n,
+                           hipStream_t s) {
+    const dim3 grid((n + 63) / 64), block(64);
+    (void)layout;
+    hipLaunchKernelGGL(debug_fe_kernel<Field10>, grid, block, 0, s, op, a, b, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_mad_rate(uint32_t* out, uint32_t lanes, uint32_t seed, uint32_t iters, hipStream_t s) {
+    hipLaunchKernelGGL(mad_rate_kernel, dim3(lanes / 256), dim3(256), 0, s, out, seed, iters);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_addr(const ShaSearch& c, const uint32_t* u, uint32_t* out, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(debug_addr_kernel, dim3((n + 63) / 64), dim3(64), 0, s, c, u, out, n);
+    return hipGetLastError();
+}
+
+}  // namespace lxv

@@ -1,0 +1,502 @@
+// The C ABI of include/lxmf_vanity_hip.h: device context, persistent walk
+// state, asynchronous launch and poll, stop, checksums, and the host helpers
+// of the search driver. Host code only; the kernels are in lxv_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lxmf_vanity_hip.h"
+#include "../../include/lxmf_vanity_hip_debug.h"
+#include "fe10.h"
+#include "host_crypto.h"
+#include "lxv_kernels.h"
+#include "lxv_walk.h"
+
+using namespace lxv;
+
+namespace {
+
+thread_local std::string g_open_error = "no error";
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct lxv_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string desc;
+    std::string err = "no error";
+    uint32_t lanes = 0, batch = 64, waves = 4, layout = LXV_LAYOUT_10X26, limbs = 10;
+    uint32_t* d_state = nullptr;
+    uint64_t* d_ns = nullptr;
+    uint32_t* d_xor = nullptr;
+    uint2* d_batch = nullptr;
+    uint32_t* d_counters = nullptr;            // [0] hit count, [1] stopped, [2..3] batches (u64)
+    lxv_hit* d_hits = nullptr;
+    uint32_t hits_cap = 0;
+    uint32_t* d_dump = nullptr;
+    size_t dump_cap = 0;                        // words
+    uint32_t* h_stop = nullptr;                 // host-mapped stop flag
+    uint32_t* d_stop = nullptr;
+    bool initialized = false;
+    bool in_flight = false;
+    lxv_launch_args last{};
+    lxv_launch_stats stats{};
+};
+
+static int fail(lxv_ctx* c, int code, const std::string& msg) {
+    if (c)
+        c->err = msg;
+    else
+        g_open_error = msg;
+    return code;
+}
+
+static int hip_fail(lxv_ctx* c, hipError_t e, const char* what) {
+    return fail(c, LXV_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(c, expr)                                   \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return hip_fail(c, e_, #expr); \
+    } while (0)
+
+static void free_ctx(lxv_ctx* c) {
+    if (!c) return;
+    if (c->device >= 0) (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->d_state);
+    (void)hipFree(c->d_ns);
+    (void)hipFree(c->d_xor);
+    (void)hipFree(c->d_batch);
+    (void)hipFree(c->d_counters);
+    (void)hipFree(c->d_hits);
+    (void)hipFree(c->d_dump);
+    if (c->h_stop) hipHostFree(c->h_stop);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int lxv_open(const lxv_open_opts* opts, lxv_ctx** out) {
+    if (!out) return fail(nullptr, LXV_E_ARG, "lxv_open: out is NULL");
+    *out = nullptr;
+    lxv_open_opts o{};
+    if (opts) {
+        if (opts->size < sizeof(lxv_open_opts)) return fail(nullptr, LXV_E_ARG, "lxv_open: opts->size too small");
+        o = *opts;
+    }
+    auto* c = new (std::nothrow) lxv_ctx();
+    if (!c) return fail(nullptr, LXV_E_NOMEM, "lxv_open: out of host memory");
+    c->device = o.device < 0 ? 0 : o.device;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) {
+        delete c;
+        return fail(nullptr, LXV_E_HIP, "no HIP device found (is this an MI355X host?)");
+    }
+    if (c->device >= ndev) {
+        delete c;
+        return fail(nullptr, LXV_E_ARG, "lxv_open: no such device");
+    }
+    hipDeviceProp_t prop;
+    if ((e = hipSetDevice(c->device)) != hipSuccess || (e = hipGetDeviceProperties(&prop, c->device)) != hipSuccess) {
+        delete c;
+        return fail(nullptr, LXV_E_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+    }
+    c->layout = o.layout;
+    if (c->layout != LXV_LAYOUT_10X26 && c->layout != LXV_LAYOUT_8X32) {
+        delete c;
+        return fail(nullptr, LXV_E_ARG, "lxv_open: unknown layout");
+    }
+    c->batch = o.batch ? o.batch : 64;
+    uint32_t wpc = o.waves_per_cu ? o.waves_per_cu : 16;
+    c->waves = wpc >= 16 ? 4 : (wpc >= 8 ? 2 : 1);
+    if (!walk_supported((int)c->layout, (int)c->batch, (int)c->waves)) {
+        delete c;
+        return fail(nullptr, LXV_E_ARG,
+                    "lxv_open: no walk kernel built for this layout / batch / waves (batch must be 16..256, a power of two)");
+    }
+    c->limbs = (uint32_t)limbs_of((int)c->layout);
+    c->lanes = o.threads ? o.threads : (uint32_t)prop.multiProcessorCount * wpc * 64u;
+    c->lanes = (c->lanes + 63) / 64 * 64;
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s (%s, %d CUs, HIP)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    c->desc = buf;
+
+#define ALLOC(ptr, bytes)                                               \
+    do {                                                                \
+        hipError_t ea = hipMalloc((void**)&(ptr), (bytes));              \
+        if (ea != hipSuccess) {                                         \
+            std::string m = std::string("hipMalloc ") + #ptr + ": " + hipGetErrorString(ea); \
+            free_ctx(c);                                                \
+            return fail(nullptr, LXV_E_HIP, m);                         \
+        }                                                               \
+    } while (0)
+    const size_t L = c->lanes;
+    ALLOC(c->d_state, L * 4 * c->limbs * sizeof(uint32_t));
+    ALLOC(c->d_ns, L * sizeof(uint64_t));
+    ALLOC(c->d_xor, L * 4 * sizeof(uint32_t));
+    ALLOC(c->d_batch, L * (size_t)c->batch * 2 * c->limbs * sizeof(uint32_t));
+    ALLOC(c->d_counters, 16 * sizeof(uint32_t));
+#undef ALLOC
+    if ((e = hipHostMalloc((void**)&c->h_stop, 64, hipHostMallocMapped | hipHostMallocCoherent)) != hipSuccess) {
+        std::string m = std::string("hipHostMalloc stop flag: ") + hipGetErrorString(e);
+        free_ctx(c);
+        return fail(nullptr, LXV_E_HIP, m);
+    }
+    *(volatile uint32_t*)c->h_stop = 0;
+    if ((e = hipHostGetDevicePointer((void**)&c->d_stop, c->h_stop, 0)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess) {
+        std::string m = std::string("stream/event setup: ") + hipGetErrorString(e);
+        free_ctx(c);
+        return fail(nullptr, LXV_E_HIP, m);
+    }
+    *out = c;
+    return LXV_OK;
+}
+
+extern "C" const char* lxv_describe(lxv_ctx* c) { return c ? c->desc.c_str() : ""; }
+extern "C" uint32_t lxv_threads(lxv_ctx* c) { return c ? c->lanes : 0; }
+extern "C" uint64_t lxv_batch(lxv_ctx* c) { return c ? c->batch : 0; }
+
+static int check_idle(lxv_ctx* c, const char* what) {
+    if (!c) return LXV_E_ARG;
+    if (c->in_flight) return fail(c, LXV_E_BUSY, std::string(what) + ": a launch has not been polled");
+    return LXV_OK;
+}
+
+extern "C" int lxv_init(lxv_ctx* c, const uint32_t* upc, size_t n_words) {
+    int rc = check_idle(c, "lxv_init");
+    if (rc) return rc;
+    if (!upc || n_words != (size_t)c->lanes * 16) return fail(c, LXV_E_ARG, "lxv_init: need 16 words per lane");
+    HIPCHK(c, hipSetDevice(c->device));
+    uint32_t* d_upc = nullptr;
+    HIPCHK(c, hipMalloc((void**)&d_upc, n_words * sizeof(uint32_t)));
+    hipError_t e = hipMemcpyAsync(d_upc, upc, n_words * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_init_from_u((int)c->layout, c->d_state, c->d_ns, c->d_xor, d_upc, c->lanes, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_upc);
+    if (e != hipSuccess) return hip_fail(c, e, "lxv_init");
+    c->initialized = true;
+    return LXV_OK;
+}
+
+extern "C" int lxv_init_seeded(lxv_ctx* c, const uint8_t start[32], uint64_t stride) {
+    int rc = check_idle(c, "lxv_init_seeded");
+    if (rc) return rc;
+    if (!start || !start_scalar_ok(start)) return fail(c, LXV_E_ARG, "lxv_init_seeded: not a valid start scalar");
+    if (stride == 0 || (c->lanes > 1 && stride > (UINT64_MAX >> 1) / c->lanes))
+        return fail(c, LXV_E_ARG, "lxv_init_seeded: lanes x stride must stay below 2^63");
+    // base = start / 8 - 1 as little-endian words
+    uint8_t s8[32];
+    uint32_t carry = 0;
+    for (int i = 31; i >= 0; i--) {
+        s8[i] = (uint8_t)((start[i] >> 3) | (carry << 5));
+        carry = start[i] & 7;
+    }
+    uint32_t base[8];
+    for (int i = 0; i < 8; i++)
+        base[i] = (uint32_t)s8[4 * i] | (uint32_t)s8[4 * i + 1] << 8 | (uint32_t)s8[4 * i + 2] << 16 |
+                  (uint32_t)s8[4 * i + 3] << 24;
+    uint64_t borrow = 1;
+    for (int i = 0; i < 8 && borrow; i++) {
+        borrow = base[i] == 0;
+        base[i] -= 1;
+    }
+    lxv_launch_args a{};
+    uint8_t zero[32] = {0};
+    lxv_launch_args_for(zero, &a);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, launch_init_ladder((int)c->layout, c->d_state, c->d_ns, c->d_xor, base, a.q, stride, c->lanes, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->initialized = true;
+    return LXV_OK;
+}
+
+template <class F>
+static void limbs_of_q(const uint32_t q[8], uint32_t qp[10], uint32_t qm[10]) {
+    typedef typename F::fe fe;
+    const fe qf = F::from_words(q);
+    uint32_t a[F::kLimbs], b[F::kLimbs];
+    F::store(F::add(qf, F::one()), a);
+    F::store(F::sub(qf, F::one()), b);
+    for (int i = 0; i < 10; i++) {
+        qp[i] = i < F::kLimbs ? a[i] : 0;
+        qm[i] = i < F::kLimbs ? b[i] : 0;
+    }
+}
+
+static void fill_q(uint32_t layout, const uint32_t q[8], uint32_t qp[10], uint32_t qm[10]) {
+    (void)layout;
+    limbs_of_q<Field10>(q, qp, qm);
+}
+
+extern "C" int lxv_launch(lxv_ctx* c, const lxv_launch_args* a) {
+    int rc = check_idle(c, "lxv_launch");
+    if (rc) return rc;
+    if (!a) return fail(c, LXV_E_ARG, "lxv_launch: args is NULL");
+    if (!c->initialized) return fail(c, LXV_E_STATE, "lxv_launch: lanes not initialised (lxv_init first)");
+    if (a->dump_threads > c->lanes) return fail(c, LXV_E_ARG, "lxv_launch: dump_threads exceeds lanes");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (a->max_hits > c->hits_cap) {
+        (void)hipFree(c->d_hits);
+        c->d_hits = nullptr;
+        c->hits_cap = 0;
+        HIPCHK(c, hipMalloc((void**)&c->d_hits, (size_t)a->max_hits * sizeof(lxv_hit)));
+        c->hits_cap = a->max_hits;
+    }
+    const size_t dump_words = (size_t)a->dump_threads * c->batch * 8;
+    if (dump_words > c->dump_cap) {
+        (void)hipFree(c->d_dump);
+        c->d_dump = nullptr;
+        c->dump_cap = 0;
+        HIPCHK(c, hipMalloc((void**)&c->d_dump, dump_words * sizeof(uint32_t)));
+        c->dump_cap = dump_words;
+    }
+    WalkParams p;
+    std::memset(&p, 0, sizeof p);
+    sha_search_init(p.sha, a->ed, a->name_hash, a->want, a->mask);
+    fill_q(c->layout, a->q, p.qp, p.qm);
+    p.state = c->d_state;
+    p.ns = c->d_ns;
+    p.xor_acc = c->d_xor;
+    p.batch_mem = c->d_batch;
+    p.hits = c->d_hits;
+    p.hit_count = c->d_counters;
+    p.stopped = c->d_counters + 1;
+    p.batches = (unsigned long long*)(c->d_counters + 2);
+    p.dump = c->d_dump;
+    p.stop = c->d_stop;
+    p.max_hits = a->max_hits;
+    p.dump_threads = a->dump_threads;
+    p.lanes = c->lanes;
+    p.iters = a->iters;
+    *(volatile uint32_t*)c->h_stop = 0;
+    HIPCHK(c, launch_clear(c->d_counters, 16, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, launch_walk((int)c->layout, (int)c->batch, (int)c->waves, p, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->last = *a;
+    c->in_flight = true;
+    return LXV_OK;
+}
+
+extern "C" int lxv_poll(lxv_ctx* c, int wait, lxv_hit* hits, uint32_t cap, uint32_t* n_hits, uint32_t* dump,
+                        size_t dump_words) {
+    if (!c) return LXV_E_ARG;
+    if (!c->in_flight) return fail(c, LXV_E_STATE, "lxv_poll: nothing launched");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!wait) {
+        hipError_t q = hipEventQuery(c->ev1);
+        if (q == hipErrorNotReady) return LXV_E_BUSY;
+        if (q != hipSuccess) {
+            c->in_flight = false;
+            return hip_fail(c, q, "lxv_poll: the walk kernel failed");
+        }
+    }
+    hipError_t e = hipEventSynchronize(c->ev1);
+    c->in_flight = false;
+    if (e != hipSuccess) return hip_fail(c, e, "lxv_poll: the walk kernel failed");
+    uint32_t counters[4];
+    HIPCHK(c, hipMemcpy(counters, c->d_counters, sizeof counters, hipMemcpyDeviceToHost));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    uint64_t batches = (uint64_t)counters[2] | (uint64_t)counters[3] << 32;
+    c->stats.kernel_ms = ms;
+    c->stats.keys = batches * c->batch;
+    c->stats.hits_total = counters[0];
+    c->stats.stopped = counters[1];
+    const uint32_t stored = counters[0] < c->last.max_hits ? counters[0] : c->last.max_hits;
+    const uint32_t give = stored < cap ? stored : cap;
+    if (n_hits) *n_hits = stored;
+    if (give && hits) HIPCHK(c, hipMemcpy(hits, c->d_hits, (size_t)give * sizeof(lxv_hit), hipMemcpyDeviceToHost));
+    const size_t dw = (size_t)c->last.dump_threads * c->batch * 8;
+    if (dw && dump) {
+        if (dump_words < dw) return fail(c, LXV_E_ARG, "lxv_poll: dump buffer too small");
+        HIPCHK(c, hipMemcpy(dump, c->d_dump, dw * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
+    return LXV_OK;
+}
+
+extern "C" int lxv_stop(lxv_ctx* c) {
+    if (!c) return LXV_E_ARG;
+    __atomic_store_n(c->h_stop, 1u, __ATOMIC_SEQ_CST);
+    return LXV_OK;
+}
+
+extern "C" int lxv_checksum(lxv_ctx* c, uint32_t xw[4]) {
+    int rc = check_idle(c, "lxv_checksum");
+    if (rc) return rc;
+    if (!xw) return fail(c, LXV_E_ARG, "lxv_checksum: NULL output");
+    std::vector<uint32_t> h((size_t)4 * c->lanes);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpy(h.data(), c->d_xor, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 4; k++) {
+        uint32_t x = 0;
+        for (size_t i = 0; i < c->lanes; i++) x ^= h[(size_t)k * c->lanes + i];
+        xw[k] = x;
+    }
+    return LXV_OK;
+}
+
+extern "C" int lxv_read_ns(lxv_ctx* c, uint32_t first, uint32_t count, uint64_t* ns) {
+    int rc = check_idle(c, "lxv_read_ns");
+    if (rc) return rc;
+    if (!ns || (uint64_t)first + count > c->lanes) return fail(c, LXV_E_ARG, "lxv_read_ns: range out of bounds");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpy(ns, c->d_ns + first, (size_t)count * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return LXV_OK;
+}
+
+extern "C" int lxv_last_stats(lxv_ctx* c, lxv_launch_stats* out) {
+    if (!c || !out) return LXV_E_ARG;
+    *out = c->stats;
+    return LXV_OK;
+}
+
+extern "C" void lxv_close(lxv_ctx* c) { free_ctx(c); }
+
+extern "C" const char* lxv_last_error(lxv_ctx* c) { return c ? c->err.c_str() : g_open_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Host helpers of the search driver.
+
+extern "C" void lxv_x25519_base(const uint8_t k[32], uint8_t out[32]) { x25519_base(k, out); }
+extern "C" void lxv_ed25519_public(const uint8_t seed[32], uint8_t out[32]) { ed25519_public(seed, out); }
+extern "C" void lxv_address_of_identity(const uint8_t id[64], uint8_t out[16]) { address_of_identity(id, out); }
+extern "C" void lxv_step_scalar(const uint8_t k[32], uint64_t n, uint8_t out[32]) { step_scalar(k, n, out); }
+extern "C" int lxv_start_scalar_ok(const uint8_t k[32]) { return start_scalar_ok(k) ? 1 : 0; }
+
+extern "C" int lxv_random_start_scalar(uint8_t out[32]) {
+    for (;;) {
+        if (!random_bytes(out, 32)) return LXV_E_ARG;
+        clamp(out);
+        if (start_scalar_ok(out)) return LXV_OK;
+    }
+}
+
+extern "C" int lxv_prefix_words(const char* s, uint32_t want[4], uint32_t mask[4]) {
+    Prefix p;
+    if (!s || !Prefix::parse(s, &p, nullptr)) return LXV_E_ARG;
+    p.words(want, mask);
+    return p.nibbles;
+}
+
+static uint32_t be_word(const uint8_t* b) {
+    return (uint32_t)b[0] << 24 | (uint32_t)b[1] << 16 | (uint32_t)b[2] << 8 | b[3];
+}
+
+extern "C" void lxv_launch_args_for(const uint8_t ed_pub[32], lxv_launch_args* a) {
+    for (int i = 0; i < 8; i++) a->ed[i] = be_word(ed_pub + 4 * i);
+    uint8_t nh[10];
+    name_hash(nh);
+    a->name_hash[0] = be_word(nh);
+    a->name_hash[1] = be_word(nh + 4);
+    a->name_hash[2] = (uint32_t)nh[8] << 24 | (uint32_t)nh[9] << 16;
+    uint8_t eight[32] = {8}, base[32] = {9}, q[32];
+    mont_mul(eight, base, q);
+    for (int i = 0; i < 8; i++)
+        a->q[i] = (uint32_t)q[4 * i] | (uint32_t)q[4 * i + 1] << 8 | (uint32_t)q[4 * i + 2] << 16 |
+                  (uint32_t)q[4 * i + 3] << 24;
+}
+
+extern "C" void lxv_seeded_search(uint64_t seed, int nibbles, uint8_t start[32], uint8_t ed_seed[32], char* prefix) {
+    if (start) seeded_start(seed, start);
+    if (ed_seed) seeded_ed_seed(seed, ed_seed);
+    if (prefix) {
+        std::string p = seeded_prefix(seed, nibbles);
+        std::memcpy(prefix, p.c_str(), p.size() + 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Debug entry points.
+
+extern "C" int lxv_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n,
+                            int on_gpu) {
+    if (!a || !b || !out || op < 0 || op > 6) return LXV_E_ARG;
+    if (!on_gpu) {
+        for (uint32_t i = 0; i < n; i++) debug_fe_op<Field10>(op, a + 8 * i, b + 8 * i, out + 8 * i);
+        return LXV_OK;
+    }
+    if (n == 0) return LXV_OK;
+    uint32_t *da = nullptr, *db = nullptr, *dout = nullptr;
+    const size_t bytes = (size_t)n * 8 * sizeof(uint32_t);
+    hipError_t e = hipSetDevice(0);
+    if (e == hipSuccess) e = hipMalloc((void**)&da, bytes);
+    if (e == hipSuccess) e = hipMalloc((void**)&db, bytes);
+    if (e == hipSuccess) e = hipMalloc((void**)&dout, bytes);
+    if (e == hipSuccess) e = hipMemcpy(da, a, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db, b, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_debug_fe(layout, op, da, db, dout, n, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dout);
+    return e == hipSuccess ? LXV_OK : LXV_E_HIP;
+}
+
+extern "C" int lxv_debug_address(const uint32_t ed[8], const uint32_t nh[3], const uint32_t* u, uint32_t* out,
+                                 uint32_t n, int on_gpu) {
+    if (!ed || !nh || !u || !out) return LXV_E_ARG;
+    ShaSearch c;
+    const uint32_t zero[4] = {0, 0, 0, 0};
+    sha_search_init(c, ed, nh, zero, zero);
+    if (!on_gpu) {
+        for (uint32_t i = 0; i < n; i++) lxmf_address_words(u + 8 * i, c, out + 4 * i);
+        return LXV_OK;
+    }
+    if (n == 0) return LXV_OK;
+    uint32_t *du = nullptr, *dout = nullptr;
+    hipError_t e = hipSetDevice(0);
+    if (e == hipSuccess) e = hipMalloc((void**)&du, (size_t)n * 8 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&dout, (size_t)n * 4 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemcpy(du, u, (size_t)n * 8 * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_debug_addr(c, du, dout, n, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)n * 4 * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    (void)hipFree(du);
+    (void)hipFree(dout);
+    return e == hipSuccess ? LXV_OK : LXV_E_HIP;
+}
+
+extern "C" int lxv_debug_mad_rate(int device, double* mads_per_s, double* clock_hz, int* cus) {
+    if (!mads_per_s) return LXV_E_ARG;
+    hipDeviceProp_t prop;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) return LXV_E_HIP;
+    const uint32_t lanes = (uint32_t)prop.multiProcessorCount * 32u * 64u, iters = 8192;
+    uint32_t* out = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc((void**)&out, (size_t)lanes * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_mad_rate(out, lanes, 1, iters, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    for (int r = 0; r < 5 && e == hipSuccess; r++) e = launch_mad_rate(out, lanes, 2 + r, iters, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    if (e != hipSuccess || ms <= 0) return LXV_E_HIP;
+    *mads_per_s = 5.0 * lanes * (double)iters * 8.0 / (ms * 1e-3);
+    if (clock_hz) *clock_hz = prop.clockRate * 1e3;
+    if (cus) *cus = prop.multiProcessorCount;
+    return LXV_OK;
+}

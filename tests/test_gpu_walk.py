@@ -1,0 +1,202 @@
+"""GPU parity of the walk kernel against the CPU oracle, through the C ABI.
+
+Every (lane, n, address) hit the GPU reports, every dumped public key and the
+XOR checksum of every walked address must match the oracle bit for bit on the
+same seeded inputs. Lane t walks keys start + 8 (t x stride + n).
+"""
+
+import random
+import time
+
+import pytest
+
+import oracle_lib as orc
+import textbook as tb
+import lxv
+
+pytestmark = pytest.mark.gpu
+
+P = tb.P
+STRIDE = 1 << 20
+
+
+def words(v):
+    return [(v >> (32 * i)) & 0xffffffff for i in range(8)]
+
+
+def from_words(ws):
+    return sum(w << (32 * i) for i, w in enumerate(ws))
+
+
+def seeded(seed, nibbles):
+    start, ed_seed, prefix = lxv.seeded_search(seed, nibbles)
+    return start, ed_seed, lxv.ed25519_public(ed_seed), prefix
+
+
+def test_gpu_field_ops_match_big_integers():
+    rng = random.Random(31)
+    vals = [0, 1, 2, 19, P - 1, P, P + 1, 2**255 - 1, 2**254] + [rng.getrandbits(255) for _ in range(500)]
+    vals = [v & (2**255 - 1) for v in vals]
+    bs = [rng.choice(vals) for _ in vals]
+    A = sum((words(v) for v in vals), [])
+    B = sum((words(v) for v in bs), [])
+    expect = {0: [a * b % P for a, b in zip(vals, bs)], 1: [a * a % P for a in vals],
+              2: [(a + b) % P for a, b in zip(vals, bs)], 3: [(a - b) % P for a, b in zip(vals, bs)],
+              5: [a % P for a in vals], 6: [a * 121665 % P for a in vals],
+              4: [pow(a, P - 2, P) for a in vals]}
+    for op, want in expect.items():
+        print("field op", op, flush=True)
+        got = lxv.debug_fe(op, A, B, on_gpu=True)
+        got = [from_words(got[8 * i:8 * i + 8]) for i in range(len(vals))]
+        assert got == want, op
+
+
+def test_gpu_address_hash_matches_oracle():
+    rng = random.Random(32)
+    ed_pub = bytes(rng.getrandbits(8) for _ in range(32))
+    a = lxv.launch_args_for(ed_pub)
+    us = [bytes(rng.getrandbits(8) for _ in range(32)) for _ in range(300)]
+    U = sum(([int.from_bytes(u[4 * i:4 * i + 4], "little") for i in range(8)] for u in us), [])
+    out = lxv.debug_address(list(a.ed), list(a.name_hash), U, on_gpu=True)
+    for i, u in enumerate(us):
+        assert lxv.words_to_bytes_be(out[4 * i:4 * i + 4]) == orc.lxmf_address(u, ed_pub)
+
+
+@pytest.fixture(scope="module")
+def small_backend():
+    b = lxv.Backend(threads=4096, batch=64)
+    yield b
+    b.close()
+
+
+def test_walk_dump_matches_x25519(small_backend):
+    b = small_backend
+    start, _, ed_pub, _ = seeded(41, 2)
+    b.init_seeded(start, STRIDE)
+    hits, dump = b.launch(lxv.launch_args_for(ed_pub, "00", iters=1, dump_threads=256))
+    rng = random.Random(42)
+    lanes = [0, 1, 63, 64, 255] + rng.sample(range(256), 20)
+    for t in lanes:
+        for i in list(range(8)) + [31, 32, 62, 63]:
+            k = orc.step_scalar(start, t * STRIDE + i)
+            got = lxv.words_to_bytes_le(dump[(t * 64 + i) * 8:(t * 64 + i) * 8 + 8])
+            assert got == orc.x25519(k), (t, i)
+
+
+@pytest.mark.parametrize("iters,prefix_len", [(1, 2), (4, 3)])
+def test_walk_hits_and_checksum_match_oracle(small_backend, iters, prefix_len):
+    b = small_backend
+    start, _, ed_pub, prefix = seeded(43 + iters, prefix_len)
+    b.init_seeded(start, STRIDE)
+    hits, _ = b.launch(lxv.launch_args_for(ed_pub, prefix, iters=iters, max_hits=1 << 16))
+    per_lane = iters * b.batch
+    st = b.stats()
+    assert st.keys == b.threads * per_lane and st.stopped == 0
+    assert st.hits_total == len(hits)
+    got = sorted((t * STRIDE + n, addr) for t, n, addr in hits)
+    ranges = [(t * STRIDE, per_lane) for t in range(b.threads)]
+    want, xor = orc.scan_ranges(start, ed_pub, ranges, prefix, cap=1 << 16)
+    print("hits", len(got), "expected", len(want), flush=True)
+    assert len(want) > 10
+    assert got == [(g, a) for g, a in want]
+    assert b.checksum() == xor
+    assert b.read_ns() == [per_lane] * b.threads
+
+
+def test_walk_continues_across_launches(small_backend):
+    # Two launches of 2 batches walk the same keys as one of 4.
+    b = small_backend
+    start, _, ed_pub, prefix = seeded(47, 3)
+    b.init_seeded(start, STRIDE)
+    a = lxv.launch_args_for(ed_pub, prefix, iters=2, max_hits=1 << 16)
+    h1, _ = b.launch(a)
+    h2, _ = b.launch(a)
+    x_two = b.checksum()
+    b.init_seeded(start, STRIDE)
+    h4, _ = b.launch(lxv.launch_args_for(ed_pub, prefix, iters=4, max_hits=1 << 16))
+    assert sorted(h1 + h2) == sorted(h4)
+    assert b.checksum() == x_two
+
+
+def test_reference_style_init_from_host_start_points():
+    # Backend::init with u(k-8), u(k) per lane from independent random starts
+    # (src/gpu/mod.rs:210-228), computed with the product's host X25519.
+    b = lxv.Backend(threads=256, batch=64)
+    try:
+        rng = random.Random(48)
+        starts = [tb.clamp(bytes(rng.getrandbits(8) for _ in range(32))) for _ in range(b.threads)]
+        starts = [s for s in starts if orc.start_scalar_ok(s)]
+        assert len(starts) == b.threads
+        upc = []
+        for s in starts:
+            prev = orc.previous_scalar(s)
+            for k in (prev, s):
+                u = lxv.x25519_base(k)
+                upc += [int.from_bytes(u[4 * i:4 * i + 4], "little") for i in range(8)]
+        b.init(upc)
+        ed_pub = lxv.ed25519_public(bytes(32))
+        hits, dump = b.launch(lxv.launch_args_for(ed_pub, "0", iters=1, dump_threads=b.threads, max_hits=1 << 14))
+        for t in range(0, b.threads, 7):
+            for i in (0, 1, 33, 63):
+                got = lxv.words_to_bytes_le(dump[(t * 64 + i) * 8:(t * 64 + i) * 8 + 8])
+                assert got == orc.x25519(orc.step_scalar(starts[t], i)), (t, i)
+        for t, n, addr in hits:
+            assert addr == orc.lxmf_address(orc.x25519(orc.step_scalar(starts[t], n)), ed_pub)
+            assert addr.hex().startswith("0")
+        assert len(hits) > 0
+    finally:
+        b.close()
+
+
+def test_full_address_prefix_finds_exactly_that_key(small_backend):
+    b = small_backend
+    start, _, ed_pub, _ = seeded(49, 0)
+    t, n = 1234, 37
+    want = orc.lxmf_address(orc.x25519(orc.step_scalar(start, t * STRIDE + n)), ed_pub)
+    b.init_seeded(start, STRIDE)
+    hits, _ = b.launch(lxv.launch_args_for(ed_pub, want.hex(), iters=1))
+    assert hits == [(t, n, want)]
+
+
+def test_hits_beyond_max_hits_are_counted_not_stored(small_backend):
+    b = small_backend
+    start, _, ed_pub, _ = seeded(50, 0)
+    b.init_seeded(start, STRIDE)
+    hits, _ = b.launch(lxv.launch_args_for(ed_pub, "0", iters=1, max_hits=100))
+    st = b.stats()
+    assert len(hits) == 100 and st.hits_total > 10000
+
+
+def test_stop_ends_a_long_launch_and_the_walk_resumes(small_backend):
+    b = small_backend
+    start, _, ed_pub, prefix = seeded(51, 3)
+    b.init_seeded(start, STRIDE)
+    b.launch_async(lxv.launch_args_for(ed_pub, prefix, iters=20000, max_hits=1 << 16))
+    time.sleep(0.3)
+    b.stop()
+    hits, _ = b.poll(True)
+    st = b.stats()
+    assert st.stopped == 1 and 0 < st.keys < b.threads * 64 * 20000
+    ns = b.read_ns()
+    assert all(n % 64 == 0 for n in ns) and sum(ns) == st.keys
+    # The walk resumes where each lane stopped: the next batch's keys match.
+    h2, dump = b.launch(lxv.launch_args_for(ed_pub, prefix, iters=1, dump_threads=16))
+    for t in range(16):
+        for i in (0, 63):
+            k = orc.step_scalar(start, t * STRIDE + ns[t] + i)
+            assert lxv.words_to_bytes_le(dump[(t * 64 + i) * 8:(t * 64 + i) * 8 + 8]) == orc.x25519(k)
+    for t, n, addr in hits + h2:
+        assert addr == orc.lxmf_address(orc.x25519(orc.step_scalar(start, t * STRIDE + n)), ed_pub)
+
+
+def test_self_test_and_run():
+    start, ed_seed, _, _ = seeded(52, 0)
+    s = lxv.GpuSearch(start=start, ed_seed=ed_seed, threads=16384)
+    try:
+        report = s.self_test()
+        print(report, flush=True)
+        identity, tries = s.run("abc")
+        addr = orc.address_of_identity(identity)
+        assert addr.hex().startswith("abc") and tries > 0
+    finally:
+        s.close()

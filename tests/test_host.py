@@ -1,0 +1,175 @@
+"""CPU tests of the product library (no GPU needed).
+
+* The C-ABI library loads and exports every function include/*.h declares.
+* The host crypto of the search driver (used for setup and for re-deriving
+  hits) matches the oracle, the golden fixtures and the reference's known
+  answer.
+* The device field and hash code, compiled for the host from the same source
+  (lxv_debug_* with on_gpu=0), matches Python big integers and the oracle.
+"""
+
+import os
+import random
+import re
+
+import pytest
+
+import oracle_lib as orc
+import textbook as tb
+import lxv
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+P = tb.P
+
+
+def declared_functions():
+    names = []
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(lxv_[a-z0-9_]+)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 25, names
+    L = lxv.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_open_without_gpu_fails_cleanly():
+    if os.path.exists("/dev/kfd") and os.environ.get("HIP_VISIBLE_DEVICES", "") != "-1":
+        pytest.skip("a GPU may be present")
+    with pytest.raises(lxv.GpuUnavailable):
+        lxv.Backend()
+
+
+def test_host_crypto_known_answers():
+    r = orc_kat = __import__("json").load(open(os.path.join(ROOT, "tests", "golden", "kat.json")))
+    rns = r["rns"]
+    ident = bytes.fromhex(rns["identity"])
+    assert lxv.x25519_base(ident[:32]).hex() == rns["x_pub"]
+    assert lxv.ed25519_public(ident[32:]).hex() == rns["ed_pub"]
+    assert lxv.address_of_identity(ident).hex() == rns["address"]
+    for seed, pub in orc_kat["ed25519"]:
+        assert lxv.ed25519_public(bytes.fromhex(seed)).hex() == pub
+    for k, u, out in orc_kat["x25519"]:
+        o = ctypes_x25519(bytes.fromhex(k), bytes.fromhex(u))
+        assert o is None or o.hex() == out
+
+
+def ctypes_x25519(k, u):
+    # the product exports X25519 only against the base point
+    if u != tb.int_le(9):
+        return None
+    return lxv.x25519_base(k)
+
+
+def test_host_crypto_matches_oracle_on_random_inputs():
+    rng = random.Random(21)
+    for _ in range(30):
+        k = bytes(rng.getrandbits(8) for _ in range(32))
+        s = bytes(rng.getrandbits(8) for _ in range(32))
+        assert lxv.x25519_base(k) == orc.x25519(k)
+        assert lxv.ed25519_public(s) == orc.ed25519_pub(s)
+        assert lxv.address_of_identity(k + s) == orc.address_of_identity(k + s)
+        n = rng.getrandbits(60)
+        assert lxv.step_scalar(k, n) == orc.step_scalar(k, n)
+        assert lxv.start_scalar_ok(k) == orc.start_scalar_ok(k)
+
+
+def test_launch_args_constants():
+    a = lxv.launch_args_for(bytes(range(32)), "b00b1e5")
+    k = __import__("json").load(open(os.path.join(ROOT, "tests", "golden", "kat.json")))
+    assert lxv.words_to_bytes_le(a.q).hex() == k["u_8B"]
+    nh = bytes.fromhex(k["name_hash"])
+    assert list(a.name_hash) == [int.from_bytes(nh[0:4], "big"), int.from_bytes(nh[4:8], "big"),
+                                 nh[8] << 24 | nh[9] << 16]
+    assert lxv.words_to_bytes_be(a.ed) == bytes(range(32))
+    assert (list(a.want), list(a.mask)) == orc.prefix_words("b00b1e5")
+    with pytest.raises(ValueError):
+        lxv.prefix_words("BOOB")
+
+
+def test_seeded_workloads_match_the_textbook_generator():
+    for seed in (0, 1, 7, 2024, 2**63 + 5):
+        s, e, p = lxv.seeded_search(seed, 9)
+        assert s == tb.seeded_start(seed)
+        assert e == tb.seeded_ed_seed(seed)
+        assert p == tb.seeded_prefix(seed, 9)
+
+
+def test_random_start_scalars_are_starts():
+    for _ in range(20):
+        assert orc.start_scalar_ok(lxv.random_start_scalar())
+
+
+# ---- the device code, host build
+
+EDGE = [0, 1, 2, 19, P - 1, P, P + 1, 2**255 - 1, 2**254, 2**255 - 20, (2**26 - 1) * 2**230]
+
+
+def fe_cases(n, seed):
+    rng = random.Random(seed)
+    vals = list(EDGE) + [rng.getrandbits(255) for _ in range(n)]
+    # Decoding drops bit 255 (RFC 7748 5), so inputs are taken below 2^255.
+    return [v & (2**255 - 1) for v in vals]
+
+
+def words(v):
+    return [(v >> (32 * i)) & 0xffffffff for i in range(8)]
+
+
+def from_words(ws):
+    return sum(w << (32 * i) for i, w in enumerate(ws))
+
+
+@pytest.mark.parametrize("layout", [lxv.LAYOUT_10X26])
+def test_device_field_code_on_host(layout):
+    vals = fe_cases(200, 22)
+    rng = random.Random(23)
+    bs = [rng.choice(vals) for _ in vals]
+    A = sum((words(v) for v in vals), [])
+    B = sum((words(v) for v in bs), [])
+    expect = {
+        0: [a * b % P for a, b in zip(vals, bs)],
+        1: [a * a % P for a in vals],
+        2: [(a + b) % P for a, b in zip(vals, bs)],
+        3: [(a - b) % P for a, b in zip(vals, bs)],
+        5: [a % P for a in vals],
+        6: [a * 121665 % P for a in vals],
+    }
+    for op, want in expect.items():
+        got = lxv.debug_fe(op, A, B, on_gpu=False, layout=layout)
+        got = [from_words(got[8 * i:8 * i + 8]) for i in range(len(vals))]
+        assert got == want, op
+    inv = lxv.debug_fe(4, A[:8 * 40], B[:8 * 40], on_gpu=False, layout=layout)
+    for i, a in enumerate(vals[:40]):
+        assert from_words(inv[8 * i:8 * i + 8]) == pow(a, P - 2, P)
+
+
+def test_device_field_code_chains_on_host():
+    # Outputs fed back as inputs many times: limb bounds hold across a chain.
+    rng = random.Random(24)
+    x = rng.getrandbits(255)
+    y = rng.getrandbits(255)
+    for _ in range(200):
+        got = lxv.debug_fe(0, words(x), words(y), on_gpu=False)
+        z = from_words(got)
+        assert z == x * y % P
+        x, y = z, (z + y) % P
+
+
+def test_device_address_hash_on_host():
+    rng = random.Random(25)
+    ed_pub = bytes(rng.getrandbits(8) for _ in range(32))
+    a = lxv.launch_args_for(ed_pub)
+    us = [bytes(rng.getrandbits(8) for _ in range(32)) for _ in range(64)]
+    U = sum(([int.from_bytes(u[4 * i:4 * i + 4], "little") for i in range(8)] for u in us), [])
+    out = lxv.debug_address(list(a.ed), list(a.name_hash), U, on_gpu=False)
+    for i, u in enumerate(us):
+        assert lxv.words_to_bytes_be(out[4 * i:4 * i + 4]) == orc.lxmf_address(u, ed_pub)
