@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--layout", type=int, default=lxv.LAYOUT_10X26)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run one plain copy kernel of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build walks on a single MI355X and has no multi-GPU path "
@@ -138,9 +140,11 @@ def main():
                 "unit": "T mad/s", "frac": (achieved / mad_peak) if mad_peak else None,
                 "traffic": None, "mads_per_key": work}
     pmc = load_traffic()
-    if pmc:
-        roofline["traffic"] = pmc.get("hbm_bytes_per_launch")
-        roofline["traffic_note"] = pmc.get("note")
+    if pmc and "hbm_bytes_per_key" in pmc:
+        # bytes per launch of this run = measured bytes per key x keys per launch
+        roofline["traffic"] = pmc["hbm_bytes_per_key"] * per_iter * iters
+        roofline["traffic_source"] = "profiles/walk_pmc.json (FETCH_SIZE/WRITE_SIZE, copy-calibrated)"
+        roofline["valu_insts_per_key"] = pmc.get("valu_insts_per_key")
 
     out = {
         "metric": METRIC, "value": value, "unit": "keys/s", "n_gpus": 1, "steps": args.steps,
@@ -151,11 +155,18 @@ def main():
                                % (b.batch, args.budget_log2),
                    "lanes": b.threads, "keys_per_inversion": b.batch, "iters_per_step": iters,
                    "prefix": prefix, "layout": "10x25.5" if args.layout == lxv.LAYOUT_10X26 else "8x32",
-                   "device": b.describe()},
+                   "device": b.describe().strip()},
         "keys": keys, "wall_s": wall, "hits": len(hits), "hits_rechecked": rechecked,
         "roofline": roofline,
         "reference_gb10_keys_per_s": 630e6,
     }
+    if args.copy_probe_mib:
+        import ctypes
+        L = lxv.lib()
+        L.lxv_debug_copy.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+        ms = ctypes.c_double(0)
+        L.lxv_debug_copy(0, args.copy_probe_mib << 20, ctypes.byref(ms))
+        out["copy_probe"] = {"bytes": args.copy_probe_mib << 20, "ms": ms.value}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

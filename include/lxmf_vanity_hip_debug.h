@@ -42,6 +42,11 @@ int lxv_debug_address(const uint32_t ed[8], const uint32_t nh[3], const uint32_t
  * device's clock and CU count. */
 int lxv_debug_mad_rate(int device, double* mads_per_s, double* clock_hz, int* cus);
 
+/* Runs one plain device-to-device copy kernel of `bytes` (a multiple of 16)
+ * on `device` and returns its time: the known-size reference that HBM byte
+ * counters of the walk are scaled by. */
+int lxv_debug_copy(int device, uint64_t bytes, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,8 @@ hipError_t launch_debug_fe(int layout, int op, const uint32_t* a, const uint32_t
                            hipStream_t s);
 // lanes must be a multiple of 256; 8 mads per lane per iteration.
 hipError_t launch_mad_rate(uint32_t* out, uint32_t lanes, uint32_t seed, uint32_t iters, hipStream_t s);
+// Copies bytes (a multiple of 16) from src to dst.
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_debug_addr(const ShaSearch& c, const uint32_t* u, uint32_t* out, uint32_t n, hipStream_t s);
 
 }  // namespace lxv

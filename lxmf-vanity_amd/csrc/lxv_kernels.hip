@@ -92,6 +92,12 @@ __global__ __launch_bounds__(256) void mad_rate_kernel(uint32_t* out, uint32_t s
     out[lane] = (uint32_t)(w0 ^ w1 ^ w2 ^ w3 ^ w4 ^ w5 ^ w6 ^ (w7 >> 3));
 }
 
+// A plain copy of known size, for scaling HBM byte counters.
+__global__ __launch_bounds__(256) void copy_kernel(const uint4* src, uint4* dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 
@@ -172,6 +178,11 @@ hipError_t launch_debug_fe(int layout, int op, const uint32_t* a, const uint32_t
 
 hipError_t launch_mad_rate(uint32_t* out, uint32_t lanes, uint32_t seed, uint32_t iters, hipStream_t s) {
     hipLaunchKernelGGL(mad_rate_kernel, dim3(lanes / 256), dim3(256), 0, s, out, seed, iters);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, bytes / 16);
     return hipGetLastError();
 }
 

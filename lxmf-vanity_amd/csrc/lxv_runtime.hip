@@ -131,7 +131,7 @@ extern "C" int lxv_open(const lxv_open_opts* opts, lxv_ctx** out) {
     c->lanes = o.threads ? o.threads : (uint32_t)prop.multiProcessorCount * wpc * 64u;
     c->lanes = (c->lanes + 63) / 64 * 64;
     char buf[256];
-    snprintf(buf, sizeof buf, "%s (%s, %d CUs, HIP)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    snprintf(buf, sizeof buf, "%s%s%s, %d CUs (HIP)", prop.name, prop.name[0] ? " " : "AMD GPU ", prop.gcnArchName, prop.multiProcessorCount);
     c->desc = buf;
 
 #define ALLOC(ptr, bytes)                                               \
@@ -499,4 +499,29 @@ extern "C" int lxv_debug_mad_rate(int device, double* mads_per_s, double* clock_
     if (clock_hz) *clock_hz = prop.clockRate * 1e3;
     if (cus) *cus = prop.multiProcessorCount;
     return LXV_OK;
+}
+
+extern "C" int lxv_debug_copy(int device, uint64_t bytes, double* ms) {
+    if (bytes == 0 || bytes % 16) return LXV_E_ARG;
+    void *src = nullptr, *dst = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipMalloc(&src, bytes);
+    if (e == hipSuccess) e = hipMalloc(&dst, bytes);
+    if (e == hipSuccess) e = hipMemset(src, 1, bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    if (e == hipSuccess) e = launch_copy(src, dst, bytes, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (ms) *ms = t;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    return e == hipSuccess ? LXV_OK : LXV_E_HIP;
 }
