@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--budget-log2", type=int, default=36)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--waves-per-cu", type=int, default=0)
-    ap.add_argument("--layout", type=int, default=lxv.LAYOUT_10X26)
+    ap.add_argument("--layout", type=int, default=lxv.LAYOUT_8X32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one plain copy kernel of this many MiB (counter calibration)")
@@ -149,7 +149,7 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "keys/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": sum(kernel_ms) / len(kernel_ms), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "int32 (exact integer; i64 column sums)",
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32 (exact integer limbs)",
         "data": "synthetic (seeded start scalar, Ed25519 key and prefix)",
         "config": {"workload": "plain HIP walk, single search, 7-hex prefix, B=%d, 2^%d-key budget"
                                % (b.batch, args.budget_log2),

@@ -56,7 +56,7 @@ typedef struct lxv_open_opts {
     uint32_t threads;      /* walk lanes; 0 = CUs x waves_per_cu x 64 */
     uint32_t batch;        /* keys per batched inversion (B); 0 = 64 */
     uint32_t waves_per_cu; /* resident waves per CU the grid is sized for; 0 = 8 */
-    uint32_t layout;       /* LXV_LAYOUT_*; the 10x26 layout supports B <= 256 */
+    uint32_t layout;       /* LXV_LAYOUT_*; 0 (8x32) is the default */
 } lxv_open_opts;
 
 /* What one launch needs besides the per-lane state: LaunchArgs,

@@ -11,6 +11,7 @@
 //   debug kernels  field operations and the address hash on given inputs, for
 //                  parity tests against the CPU
 #include "fe10.h"
+#include "fe8.h"
 #include "lxv_kernels.h"
 #include "lxv_walk.h"
 
@@ -116,7 +117,9 @@ static const WalkEntry kWalks[] = {
 #define LXV_W3(F, LAYOUT, B) LXV_WALK_ENTRY(F, LAYOUT, B, 1), LXV_WALK_ENTRY(F, LAYOUT, B, 2), LXV_WALK_ENTRY(F, LAYOUT, B, 4)
     LXV_W3(Field10, LXV_LAYOUT_10X26, 16),  LXV_W3(Field10, LXV_LAYOUT_10X26, 32),
     LXV_W3(Field10, LXV_LAYOUT_10X26, 64),  LXV_W3(Field10, LXV_LAYOUT_10X26, 128),
-    LXV_W3(Field10, LXV_LAYOUT_10X26, 256),
+    LXV_W3(Field10, LXV_LAYOUT_10X26, 256), LXV_W3(Field8, LXV_LAYOUT_8X32, 16),
+    LXV_W3(Field8, LXV_LAYOUT_8X32, 32),    LXV_W3(Field8, LXV_LAYOUT_8X32, 64),
+    LXV_W3(Field8, LXV_LAYOUT_8X32, 128),   LXV_W3(Field8, LXV_LAYOUT_8X32, 256),
 #undef LXV_W3
 };
 
@@ -126,7 +129,7 @@ bool walk_supported(int layout, int batch, int waves) {
     return false;
 }
 
-int limbs_of(int layout) { return Field10::kLimbs; (void)layout; }
+int limbs_of(int layout) { return layout == LXV_LAYOUT_8X32 ? Field8::kLimbs : Field10::kLimbs; }
 
 hipError_t launch_walk(int layout, int batch, int waves, const WalkParams& p, hipStream_t s) {
     for (const auto& e : kWalks) {
@@ -142,8 +145,10 @@ hipError_t launch_walk(int layout, int batch, int waves, const WalkParams& p, hi
 hipError_t launch_init_from_u(int layout, uint32_t* state, uint64_t* ns, uint32_t* xor_acc, const uint32_t* upc,
                               uint32_t lanes, hipStream_t s) {
     const dim3 grid((lanes + 63) / 64), block(64);
-    (void)layout;
-    hipLaunchKernelGGL(init_from_u_kernel<Field10>, grid, block, 0, s, state, ns, xor_acc, upc, lanes);
+    if (layout == LXV_LAYOUT_8X32)
+        hipLaunchKernelGGL(init_from_u_kernel<Field8>, grid, block, 0, s, state, ns, xor_acc, upc, lanes);
+    else
+        hipLaunchKernelGGL(init_from_u_kernel<Field10>, grid, block, 0, s, state, ns, xor_acc, upc, lanes);
     return hipGetLastError();
 }
 
@@ -157,8 +162,10 @@ hipError_t launch_init_ladder(int layout, uint32_t* state, uint64_t* ns, uint32_
     a.stride = stride;
     a.lanes = lanes;
     const dim3 grid((lanes + 63) / 64), block(64);
-    (void)layout;
-    hipLaunchKernelGGL(init_ladder_kernel<Field10>, grid, block, 0, s, state, ns, xor_acc, a);
+    if (layout == LXV_LAYOUT_8X32)
+        hipLaunchKernelGGL(init_ladder_kernel<Field8>, grid, block, 0, s, state, ns, xor_acc, a);
+    else
+        hipLaunchKernelGGL(init_ladder_kernel<Field10>, grid, block, 0, s, state, ns, xor_acc, a);
     return hipGetLastError();
 }
 
@@ -171,8 +178,10 @@ hipError_t launch_clear(uint32_t* words, uint32_t n, hipStream_t s) {
 hipError_t launch_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n,
                            hipStream_t s) {
     const dim3 grid((n + 63) / 64), block(64);
-    (void)layout;
-    hipLaunchKernelGGL(debug_fe_kernel<Field10>, grid, block, 0, s, op, a, b, out, n);
+    if (layout == LXV_LAYOUT_8X32)
+        hipLaunchKernelGGL(debug_fe_kernel<Field8>, grid, block, 0, s, op, a, b, out, n);
+    else
+        hipLaunchKernelGGL(debug_fe_kernel<Field10>, grid, block, 0, s, op, a, b, out, n);
     return hipGetLastError();
 }
 

@@ -11,6 +11,7 @@
 #include "../../include/lxmf_vanity_hip.h"
 #include "../../include/lxmf_vanity_hip_debug.h"
 #include "fe10.h"
+#include "fe8.h"
 #include "host_crypto.h"
 #include "lxv_kernels.h"
 #include "lxv_walk.h"
@@ -114,7 +115,7 @@ extern "C" int lxv_open(const lxv_open_opts* opts, lxv_ctx** out) {
         delete c;
         return fail(nullptr, LXV_E_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
     }
-    c->layout = o.layout;
+    c->layout = o.layout;  // LXV_LAYOUT_8X32 (0) is the default: the faster layout measured on gfx950
     if (c->layout != LXV_LAYOUT_10X26 && c->layout != LXV_LAYOUT_8X32) {
         delete c;
         return fail(nullptr, LXV_E_ARG, "lxv_open: unknown layout");
@@ -239,8 +240,10 @@ static void limbs_of_q(const uint32_t q[8], uint32_t qp[10], uint32_t qm[10]) {
 }
 
 static void fill_q(uint32_t layout, const uint32_t q[8], uint32_t qp[10], uint32_t qm[10]) {
-    (void)layout;
-    limbs_of_q<Field10>(q, qp, qm);
+    if (layout == LXV_LAYOUT_8X32)
+        limbs_of_q<Field8>(q, qp, qm);
+    else
+        limbs_of_q<Field10>(q, qp, qm);
 }
 
 extern "C" int lxv_launch(lxv_ctx* c, const lxv_launch_args* a) {
@@ -427,8 +430,14 @@ extern "C" void lxv_seeded_search(uint64_t seed, int nibbles, uint8_t start[32],
 extern "C" int lxv_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n,
                             int on_gpu) {
     if (!a || !b || !out || op < 0 || op > 6) return LXV_E_ARG;
+    if (layout != LXV_LAYOUT_10X26 && layout != LXV_LAYOUT_8X32) return LXV_E_ARG;
     if (!on_gpu) {
-        for (uint32_t i = 0; i < n; i++) debug_fe_op<Field10>(op, a + 8 * i, b + 8 * i, out + 8 * i);
+        for (uint32_t i = 0; i < n; i++) {
+            if (layout == LXV_LAYOUT_8X32)
+                debug_fe_op<Field8>(op, a + 8 * i, b + 8 * i, out + 8 * i);
+            else
+                debug_fe_op<Field10>(op, a + 8 * i, b + 8 * i, out + 8 * i);
+        }
         return LXV_OK;
     }
     if (n == 0) return LXV_OK;

@@ -184,7 +184,7 @@ def words_to_bytes_le(words):
 class Backend:
     """One GPU context: Backend::{describe, threads, batch, init, launch}."""
 
-    def __init__(self, threads=0, batch=64, waves_per_cu=0, layout=LAYOUT_10X26, device=0):
+    def __init__(self, threads=0, batch=64, waves_per_cu=0, layout=LAYOUT_8X32, device=0):
         o = OpenOpts(ctypes.sizeof(OpenOpts), device, threads, batch, waves_per_cu, layout)
         h = ctypes.c_void_p()
         rc = lib().lxv_open(ctypes.byref(o), ctypes.byref(h))
@@ -327,7 +327,7 @@ class GpuSearch:
         self.backend.close()
 
 
-def debug_fe(op, a_words, b_words, on_gpu, layout=LAYOUT_10X26):
+def debug_fe(op, a_words, b_words, on_gpu, layout=LAYOUT_8X32):
     n = len(a_words) // 8
     A = (ctypes.c_uint32 * max(len(a_words), 1))(*a_words)
     B = (ctypes.c_uint32 * max(len(b_words), 1))(*b_words)

@@ -33,9 +33,13 @@ def seeded(seed, nibbles):
     return start, ed_seed, lxv.ed25519_public(ed_seed), prefix
 
 
-def test_gpu_field_ops_match_big_integers():
+@pytest.mark.parametrize("layout", [lxv.LAYOUT_10X26, lxv.LAYOUT_8X32])
+def test_gpu_field_ops_match_big_integers(layout):
     rng = random.Random(31)
-    vals = [0, 1, 2, 19, P - 1, P, P + 1, 2**255 - 1, 2**254] + [rng.getrandbits(255) for _ in range(500)]
+    vals = [0, 1, 2, 19, P - 1, P, P + 1, 2**255 - 1, 2**254, 2**255 - 20, 2**256 - 1, 2**224 * (2**32 - 1)]
+    vals += [rng.getrandbits(255) for _ in range(500)]
+    # 8x32 limbs see inputs with every word at its maximum too (values mod 2^255)
+    vals += [(2**255 - 1) - rng.getrandbits(20) for _ in range(50)]
     vals = [v & (2**255 - 1) for v in vals]
     bs = [rng.choice(vals) for _ in vals]
     A = sum((words(v) for v in vals), [])
@@ -46,7 +50,7 @@ def test_gpu_field_ops_match_big_integers():
               4: [pow(a, P - 2, P) for a in vals]}
     for op, want in expect.items():
         print("field op", op, flush=True)
-        got = lxv.debug_fe(op, A, B, on_gpu=True)
+        got = lxv.debug_fe(op, A, B, on_gpu=True, layout=layout)
         got = [from_words(got[8 * i:8 * i + 8]) for i in range(len(vals))]
         assert got == want, op
 
@@ -62,9 +66,9 @@ def test_gpu_address_hash_matches_oracle():
         assert lxv.words_to_bytes_be(out[4 * i:4 * i + 4]) == orc.lxmf_address(u, ed_pub)
 
 
-@pytest.fixture(scope="module")
-def small_backend():
-    b = lxv.Backend(threads=4096, batch=64)
+@pytest.fixture(scope="module", params=[lxv.LAYOUT_10X26, lxv.LAYOUT_8X32], ids=["10x26", "8x32"])
+def small_backend(request):
+    b = lxv.Backend(threads=4096, batch=64, layout=request.param)
     yield b
     b.close()
 

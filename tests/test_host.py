@@ -128,7 +128,7 @@ def from_words(ws):
     return sum(w << (32 * i) for i, w in enumerate(ws))
 
 
-@pytest.mark.parametrize("layout", [lxv.LAYOUT_10X26])
+@pytest.mark.parametrize("layout", [lxv.LAYOUT_10X26, lxv.LAYOUT_8X32])
 def test_device_field_code_on_host(layout):
     vals = fe_cases(200, 22)
     rng = random.Random(23)
@@ -152,13 +152,14 @@ def test_device_field_code_on_host(layout):
         assert from_words(inv[8 * i:8 * i + 8]) == pow(a, P - 2, P)
 
 
-def test_device_field_code_chains_on_host():
+@pytest.mark.parametrize("layout", [lxv.LAYOUT_10X26, lxv.LAYOUT_8X32])
+def test_device_field_code_chains_on_host(layout):
     # Outputs fed back as inputs many times: limb bounds hold across a chain.
     rng = random.Random(24)
     x = rng.getrandbits(255)
     y = rng.getrandbits(255)
     for _ in range(200):
-        got = lxv.debug_fe(0, words(x), words(y), on_gpu=False)
+        got = lxv.debug_fe(0, words(x), words(y), on_gpu=False, layout=layout)
         z = from_words(got)
         assert z == x * y % P
         x, y = z, (z + y) % P
