@@ -47,6 +47,13 @@ int lxv_debug_mad_rate(int device, double* mads_per_s, double* clock_hz, int* cu
  * counters of the walk are scaled by. */
 int lxv_debug_copy(int device, uint64_t bytes, double* ms);
 
+/* Register and scratch use of the walk kernel built for (layout, batch,
+ * waves_per_cu): VGPRs per lane, scratch bytes per lane, and the waves per
+ * SIMD its register budget was compiled for. Needs a GPU (reads the loaded
+ * code object). */
+int lxv_debug_kernel_attrs(int layout, uint32_t batch, uint32_t waves_per_cu, uint32_t* vgprs, uint32_t* sgprs,
+                           uint32_t* scratch_bytes, uint32_t* waves_per_simd);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,168 @@
+"""Autotuner of the walk kernel: keys per inversion x waves per CU x limb layout.
+
+Each point opens the backend with that shape, sets every lane up on the GPU
+and walks a fixed budget of keys of one seeded search (an 8-hex-digit prefix
+from workload seed 2), timed with HIP events on the kernel's stream; every hit
+is re-derived on the CPU. Per point it records keys/s, the kernel's VGPRs and
+scratch bytes per lane (from its code object), and, when a PMC pass has been
+merged, VALU busy %. The fastest point is the pick; the table goes to JSON.
+
+    python autotune.py [--budget-log2 34] [--out tuned.json]
+    # VALU busy %: a second, short pass under the profiler, then a merge
+    rocprofv3 --pmc SQ_ACTIVE_INST_VALU SQ_INSTS_VALU GRBM_GUI_ACTIVE --output-format csv \\
+        -d OUT -o pmc -- python3 autotune.py --budget-log2 28 --out /dev/null
+    python autotune.py --merge-pmc OUT/pmc_counter_collection.csv --out tuned.json
+
+VALU busy % = SQ_ACTIVE_INST_VALU x 4 / (SIMDs x GRBM_GUI_ACTIVE / XCDs): the
+share of SIMD cycles issuing VALU work (a wave64 instruction occupies a
+16-lane SIMD for 4 cycles; GRBM_GUI_ACTIVE is summed over the 8 XCDs).
+"""
+
+import argparse
+import csv
+import ctypes
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import lxv  # noqa: E402
+
+BATCHES = (16, 32, 64, 128, 256)
+WAVES = (2, 4, 8, 16)
+LAYOUTS = ((lxv.LAYOUT_8X32, "8x32"), (lxv.LAYOUT_10X26, "10x25.5"))
+SIMDS_PER_CU = 4
+XCDS = 8
+
+
+def kernel_attrs(layout, batch, wpc):
+    L = lxv.lib()
+    L.lxv_debug_kernel_attrs.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32] + \
+        [ctypes.POINTER(ctypes.c_uint32)] * 4
+    v, s, sc, w = (ctypes.c_uint32() for _ in range(4))
+    rc = L.lxv_debug_kernel_attrs(layout, batch, wpc, ctypes.byref(v), ctypes.byref(s), ctypes.byref(sc),
+                                  ctypes.byref(w))
+    if rc != 0:
+        return None
+    return {"vgprs": v.value, "scratch_bytes": sc.value, "waves_per_simd": w.value}
+
+
+def run_point(layout, batch, wpc, budget, start, ed_seed, prefix):
+    s = lxv.GpuSearch(start=start, ed_seed=ed_seed, batch=batch, waves_per_cu=wpc, layout=layout)
+    try:
+        b = s.backend
+        per_iter = b.threads * b.batch
+        iters = max(1, budget // (per_iter * 2))
+        a = lxv.launch_args_for(s.ed_pub, prefix, iters=iters)
+        b.launch(lxv.launch_args_for(s.ed_pub, prefix, iters=1))   # warm-up
+        ms, keys, hits = 0.0, 0, []
+        for _ in range(2):
+            h, _ = b.launch(a)
+            st = b.stats()
+            ms += st.kernel_ms
+            keys += st.keys
+            hits += h
+        for t, n, addr in hits:
+            if lxv.address_of_identity(s.identity(t, n)) != addr or not addr.hex().startswith(prefix):
+                raise SystemExit("autotune: hit (lane %d, key %d) failed its CPU recheck" % (t, n))
+        return {"lanes": b.threads, "keys": keys, "kernel_ms": ms, "keys_per_s": keys / (ms / 1e3),
+                "hits": len(hits), "device": b.describe().strip()}
+    finally:
+        s.close()
+
+
+def sweep(budget):
+    start, ed_seed, prefix = lxv.seeded_search(2, 8)
+    points = []
+    for layout, lname in LAYOUTS:
+        for batch in BATCHES:
+            for wpc in WAVES:
+                t0 = time.time()
+                r = run_point(layout, batch, wpc, budget, start, ed_seed, prefix)
+                r.update({"layout": lname, "layout_id": layout, "batch": batch, "waves_per_cu": wpc})
+                attrs = kernel_attrs(layout, batch, wpc)
+                if attrs:
+                    r.update(attrs)
+                points.append(r)
+                print("%-8s B=%-3d waves/CU=%-2d  %.3e keys/s  vgprs=%s scratch=%s  (%.1f s)"
+                      % (lname, batch, wpc, r["keys_per_s"], r.get("vgprs"), r.get("scratch_bytes"),
+                         time.time() - t0), flush=True)
+    return {"workload": "autotune sweep: single seeded search (seed 2), 8-hex prefix %s, 2^%d keys per point"
+                        % (prefix, budget.bit_length() - 1),
+            "points": points}
+
+
+def kernel_key(name, grid):
+    # walk_kernel<lxv::Field8, 64, 4> + lanes identifies a point
+    if "walk_kernel<" not in name:
+        return None
+    inner = name.split("walk_kernel<", 1)[1].split(">", 1)[0]
+    field, batch, waves = [x.strip() for x in inner.split(",")]
+    return ("8x32" if "Field8" in field else "10x25.5", int(batch), int(waves), int(grid))
+
+
+def merge_pmc(table, csv_path):
+    acc = {}
+    for r in csv.DictReader(open(csv_path)):
+        k = kernel_key(r["Kernel_Name"], r["Grid_Size"])
+        if k is None:
+            continue
+        d = acc.setdefault(k, {})
+        d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    for p in table["points"]:
+        k = (p["layout"], p["batch"], p.get("waves_per_simd", 0), p["lanes"])
+        d = acc.get(k)
+        if not d or not d.get("GRBM_GUI_ACTIVE"):
+            continue
+        cus = p["lanes"] // (p["waves_per_cu"] * 64)
+        cycles = d["GRBM_GUI_ACTIVE"] / XCDS
+        p["valu_busy_pct"] = 100.0 * d["SQ_ACTIVE_INST_VALU"] * 4 / (cus * SIMDS_PER_CU * cycles)
+        if "SQ_INSTS_VALU" in d:
+            p["valu_insts_per_wave_key"] = d["SQ_INSTS_VALU"]
+    return table
+
+
+def pick(table):
+    best = max(table["points"], key=lambda p: p["keys_per_s"])
+    by_batch = {}
+    for p in table["points"]:
+        if p["batch"] not in by_batch or p["keys_per_s"] > by_batch[p["batch"]]["keys_per_s"]:
+            by_batch[p["batch"]] = p
+    table["pick"] = {k: best[k] for k in ("layout", "layout_id", "batch", "waves_per_cu", "keys_per_s")}
+    table["pick_by_batch"] = {str(b): {k: p[k] for k in ("layout", "layout_id", "waves_per_cu", "keys_per_s")}
+                              for b, p in sorted(by_batch.items())}
+    return table
+
+
+def load(path=os.path.join(HERE, "tuned.json")):
+    """The tuned table, or None."""
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--budget-log2", type=int, default=34)
+    ap.add_argument("--out", default=os.path.join(HERE, "tuned.json"))
+    ap.add_argument("--merge-pmc", default=None, help="PMC csv of a profiled pass to merge into --out")
+    args = ap.parse_args()
+    if args.merge_pmc:
+        table = load(args.out)
+        if table is None:
+            raise SystemExit("no table at %s to merge into" % args.out)
+        table = merge_pmc(table, args.merge_pmc)
+    else:
+        table = pick(sweep(1 << args.budget_log2))
+    if args.out != "/dev/null":
+        with open(args.out, "w") as f:
+            json.dump(table, f, indent=1)
+    print(json.dumps(table.get("pick")))
+
+
+if __name__ == "__main__":
+    main()

@@ -34,6 +34,7 @@ LXV_HD void debug_fe_op(int op, const uint32_t a[8], const uint32_t b[8], uint32
 
 bool walk_supported(int layout, int batch, int waves);
 int limbs_of(int layout);
+hipError_t walk_attributes(int layout, int batch, int waves, hipFuncAttributes* out);
 hipError_t launch_walk(int layout, int batch, int waves, const WalkParams& p, hipStream_t s);
 hipError_t launch_init_from_u(int layout, uint32_t* state, uint64_t* ns, uint32_t* xor_acc, const uint32_t* upc,
                               uint32_t lanes, hipStream_t s);

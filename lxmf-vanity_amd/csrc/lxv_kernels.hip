@@ -131,6 +131,13 @@ bool walk_supported(int layout, int batch, int waves) {
 
 int limbs_of(int layout) { return layout == LXV_LAYOUT_8X32 ? Field8::kLimbs : Field10::kLimbs; }
 
+hipError_t walk_attributes(int layout, int batch, int waves, hipFuncAttributes* out) {
+    for (const auto& e : kWalks)
+        if (e.layout == layout && e.batch == batch && e.waves == waves)
+            return hipFuncGetAttributes(out, reinterpret_cast<const void*>(e.fn));
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_walk(int layout, int batch, int waves, const WalkParams& p, hipStream_t s) {
     for (const auto& e : kWalks) {
         if (e.layout == layout && e.batch == batch && e.waves == waves) {

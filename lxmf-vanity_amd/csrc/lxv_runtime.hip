@@ -534,3 +534,16 @@ extern "C" int lxv_debug_copy(int device, uint64_t bytes, double* ms) {
     (void)hipFree(dst);
     return e == hipSuccess ? LXV_OK : LXV_E_HIP;
 }
+
+extern "C" int lxv_debug_kernel_attrs(int layout, uint32_t batch, uint32_t waves_per_cu, uint32_t* vgprs,
+                                      uint32_t* sgprs, uint32_t* scratch_bytes, uint32_t* waves_per_simd) {
+    const int w = waves_per_cu >= 16 ? 4 : (waves_per_cu >= 8 ? 2 : 1);
+    if (!walk_supported(layout, (int)batch, w)) return LXV_E_ARG;
+    hipFuncAttributes a;
+    if (walk_attributes(layout, (int)batch, w, &a) != hipSuccess) return LXV_E_HIP;
+    if (vgprs) *vgprs = (uint32_t)a.numRegs;
+    if (sgprs) *sgprs = 0;
+    if (scratch_bytes) *scratch_bytes = (uint32_t)a.localSizeBytes;
+    if (waves_per_simd) *waves_per_simd = (uint32_t)w;
+    return LXV_OK;
+}

@@ -49,12 +49,33 @@ LXV_HD constexpr uint32_t sha_kw_pad64(int t) {
     return sha_k(t) + w[t];
 }
 
-LXV_HD uint32_t sha_S0(uint32_t a) { return rotr32(a, 2) ^ rotr32(a, 13) ^ rotr32(a, 22); }
-LXV_HD uint32_t sha_S1(uint32_t e) { return rotr32(e, 6) ^ rotr32(e, 11) ^ rotr32(e, 25); }
-LXV_HD uint32_t sha_s0(uint32_t w) { return rotr32(w, 7) ^ rotr32(w, 18) ^ (w >> 3); }
-LXV_HD uint32_t sha_s1(uint32_t w) { return rotr32(w, 17) ^ rotr32(w, 19) ^ (w >> 10); }
-LXV_HD uint32_t sha_ch(uint32_t e, uint32_t f, uint32_t g) { return (e & f) ^ (~e & g); }
-LXV_HD uint32_t sha_maj(uint32_t a, uint32_t b, uint32_t c) { return (a & b) | (c & (a | b)); }
+// Three-input boolean functions in one v_bitop3_b32 each (the immediate is
+// the truth table indexed by a * 4 + b * 2 + c): XOR3 0x96, Ch 0xca, Maj 0xe8.
+LXV_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+LXV_HD uint32_t sha_ch(uint32_t e, uint32_t f, uint32_t g) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(e, f, g, 0xca);
+#else
+    return (e & f) ^ (~e & g);
+#endif
+}
+LXV_HD uint32_t sha_maj(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0xe8);
+#else
+    return (a & b) | (c & (a | b));
+#endif
+}
+LXV_HD uint32_t sha_S0(uint32_t a) { return xor3(rotr32(a, 2), rotr32(a, 13), rotr32(a, 22)); }
+LXV_HD uint32_t sha_S1(uint32_t e) { return xor3(rotr32(e, 6), rotr32(e, 11), rotr32(e, 25)); }
+LXV_HD uint32_t sha_s0(uint32_t w) { return xor3(rotr32(w, 7), rotr32(w, 18), w >> 3); }
+LXV_HD uint32_t sha_s1(uint32_t w) { return xor3(rotr32(w, 17), rotr32(w, 19), w >> 10); }
 
 // Round t on the state ring S: at round t, a..h are S[(0 - t) & 7] ..
 // S[(7 - t) & 7], so a round writes two slots and nothing moves. With t known
@@ -64,7 +85,7 @@ LXV_HD void sha_round(uint32_t S[8], int t, uint32_t kw) {
     const uint32_t e = S[(12 - (t & 7)) & 7], f = S[(13 - (t & 7)) & 7], g = S[(14 - (t & 7)) & 7];
     uint32_t& d = S[(11 - (t & 7)) & 7];
     uint32_t& h = S[(15 - (t & 7)) & 7];
-    const uint32_t t1 = h + sha_S1(e) + sha_ch(e, f, g) + kw;
+    const uint32_t t1 = (h + kw) + sha_S1(e) + sha_ch(e, f, g);
     d += t1;
     h = t1 + sha_S0(a) + sha_maj(a, b, c);
 }
