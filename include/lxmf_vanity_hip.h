@@ -191,6 +191,81 @@ void lxv_launch_args_for(const uint8_t ed_pub[32], lxv_launch_args* args);
  * first `nibbles` hex digits of a prefix (prefix needs nibbles + 1 bytes). */
 void lxv_seeded_search(uint64_t seed, int nibbles, uint8_t start[32], uint8_t ed_seed[32], char* prefix);
 
+/* ------------------------------------------------------------------------
+ * Batched searches: many independent searches (each its own start scalar,
+ * Ed25519 key and prefix) in one persistent kernel. No counterpart in the
+ * reference (one search per process); the serving loop is built on these.
+ *
+ * A search owns V virtual lanes (64 to 262144, by prefix length: about
+ * 16^digits / 2^14) whose walk state stays resident in an HBM pool; virtual
+ * lane v walks keys start + 8 (v x 2^40 + n) and all of them advance
+ * together in blocks of B keys. Each step relaunches the kernel (a captured
+ * hipGraph) over a list of work units (search, 64 virtual lanes, blocks);
+ * physical waves take units from a device queue, so lanes of searches that
+ * are done move on to unfinished ones. A search's answer is its first hit in
+ * walk order: the lowest block with a hit, then the lowest (n, v) in it. Every
+ * block up to that one is walked completely before the answer is reported, so
+ * it does not depend on what else runs. Searches are admitted in submission
+ * order as slots and pool memory allow.
+ */
+
+typedef struct lxv_multi lxv_multi;
+
+typedef struct lxv_multi_opts {
+    uint32_t size;          /* sizeof(lxv_multi_opts) */
+    int32_t device;         /* HIP device ordinal; negative means 0 */
+    uint32_t max_searches;  /* slots resident at once; 0 = 1024 */
+    uint32_t batch;         /* keys per inversion: 64 or 256; 0 = 256 */
+    uint32_t layout;        /* LXV_LAYOUT_*: 256 supports both, 64 only 8x32 */
+    uint32_t no_graph;      /* 1: launch eagerly instead of replaying a hipGraph */
+    uint64_t pool_bytes;    /* device bytes for resident walk state; 0 = 2 GiB */
+    uint32_t max_vlanes;    /* cap on a search's virtual lanes; 0 = 262144 */
+    uint32_t reserved;
+} lxv_multi_opts;
+
+typedef struct lxv_search_spec {
+    uint8_t start[32];      /* first private key of the search (as random_start_scalar) */
+    uint8_t ed_pub[32];     /* Ed25519 public key of the identity */
+    char prefix[33];        /* 1-32 hex digits, NUL-terminated */
+    uint8_t reserved[7];
+    uint64_t tag;           /* caller's cookie, returned with the result */
+    uint64_t max_keys;      /* give up after about this many keys; 0 = no limit */
+} lxv_search_spec;
+
+typedef struct lxv_search_result {
+    uint64_t id;            /* from lxv_multi_submit */
+    uint64_t tag;
+    uint64_t g;             /* step index of the answer: key = start + 8 g */
+    uint32_t address[4];    /* its LXMF address, big-endian words */
+    uint64_t keys;          /* keys this search walked to reach its answer */
+    uint32_t vlanes;        /* its virtual lanes (V) */
+    int32_t status;         /* 1 found, 0 gave up (max_keys), -1 cancelled */
+    double submit_s;        /* seconds since lxv_multi_open at submission */
+    double admit_s;         /* ... when it got its slot and lanes */
+    double done_s;          /* ... when the answer was known on the host */
+} lxv_search_result;
+
+typedef struct lxv_multi_stat {
+    uint64_t keys;          /* keys walked so far, all searches */
+    double busy_s;          /* GPU time in the walk kernel (HIP events) */
+    double last_step_ms;    /* walk kernel time of the last step */
+    uint32_t resident;      /* searches holding a slot */
+    uint32_t queued;        /* searches waiting for a slot */
+    uint32_t done;          /* searches finished (found, gave up or cancelled) */
+    uint32_t steps;         /* relaunches so far */
+} lxv_multi_stat;
+
+int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out);
+/* Queues a search; returns its id. */
+int lxv_multi_submit(lxv_multi* m, const lxv_search_spec* spec, uint64_t* id);
+/* Admits what fits, relaunches the kernel once over about budget_ms of work,
+ * and returns up to cap finished searches (more are kept for the next call). */
+int lxv_multi_step(lxv_multi* m, uint32_t budget_ms, lxv_search_result* out, uint32_t cap, uint32_t* n_out);
+int lxv_multi_cancel(lxv_multi* m, uint64_t id);
+int lxv_multi_stats(lxv_multi* m, lxv_multi_stat* out);
+void lxv_multi_close(lxv_multi* m);
+const char* lxv_multi_last_error(lxv_multi* m);
+
 #ifdef __cplusplus
 }
 #endif

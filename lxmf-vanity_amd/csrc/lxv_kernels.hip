@@ -14,6 +14,7 @@
 #include "fe8.h"
 #include "lxv_kernels.h"
 #include "lxv_walk.h"
+#include "lxv_multi.h"
 
 namespace lxv {
 
@@ -99,6 +100,27 @@ __global__ __launch_bounds__(256) void copy_kernel(const uint4* src, uint4* dst,
         dst[i] = src[i];
 }
 
+template <class F, int B>
+__global__ __launch_bounds__(64, 4) void multi_walk_kernel(const MultiParams p) {
+    const uint32_t plane = blockIdx.x * 64u + threadIdx.x;
+    if (plane < p.phys_lanes) multi_wave<F, B>(p, plane);
+}
+
+struct MultiInitArgs {
+    uint32_t base[8];
+    uint32_t q[8];
+    uint64_t stride;
+    uint32_t pool_lanes;
+    uint32_t state_off;
+    uint32_t vlanes;
+};
+
+template <class F>
+__global__ __launch_bounds__(64) void multi_init_kernel(uint32_t* pool, const MultiInitArgs a) {
+    const uint32_t v = blockIdx.x * 64u + threadIdx.x;
+    if (v < a.vlanes) multi_init_lane<F>(pool, a.pool_lanes, a.state_off, a.base, a.stride, a.q, v);
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 
@@ -173,6 +195,53 @@ hipError_t launch_init_ladder(int layout, uint32_t* state, uint64_t* ns, uint32_
         hipLaunchKernelGGL(init_ladder_kernel<Field8>, grid, block, 0, s, state, ns, xor_acc, a);
     else
         hipLaunchKernelGGL(init_ladder_kernel<Field10>, grid, block, 0, s, state, ns, xor_acc, a);
+    return hipGetLastError();
+}
+
+typedef void (*multi_fn)(const MultiParams);
+struct MultiEntry {
+    int layout;
+    int batch;
+    multi_fn fn;
+};
+static const MultiEntry kMultis[] = {
+    {LXV_LAYOUT_8X32, 64, multi_walk_kernel<Field8, 64>},
+    {LXV_LAYOUT_8X32, 256, multi_walk_kernel<Field8, 256>},
+    {LXV_LAYOUT_10X26, 256, multi_walk_kernel<Field10, 256>},
+};
+
+bool multi_supported(int layout, int batch) {
+    for (const auto& e : kMultis)
+        if (e.layout == layout && e.batch == batch) return true;
+    return false;
+}
+
+hipError_t launch_multi(int layout, int batch, const MultiParams& p, hipStream_t s) {
+    for (const auto& e : kMultis) {
+        if (e.layout == layout && e.batch == batch) {
+            hipLaunchKernelGGL(e.fn, dim3((p.phys_lanes + 63) / 64), dim3(64), 0, s, p);
+            return hipGetLastError();
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_multi_init(int layout, uint32_t* pool, uint32_t pool_lanes, uint32_t state_off, uint32_t vlanes,
+                             const uint32_t base[8], const uint32_t q[8], uint64_t stride, hipStream_t s) {
+    MultiInitArgs a;
+    for (int i = 0; i < 8; i++) {
+        a.base[i] = base[i];
+        a.q[i] = q[i];
+    }
+    a.stride = stride;
+    a.pool_lanes = pool_lanes;
+    a.state_off = state_off;
+    a.vlanes = vlanes;
+    const dim3 grid((vlanes + 63) / 64), block(64);
+    if (layout == LXV_LAYOUT_8X32)
+        hipLaunchKernelGGL(multi_init_kernel<Field8>, grid, block, 0, s, pool, a);
+    else
+        hipLaunchKernelGGL(multi_init_kernel<Field10>, grid, block, 0, s, pool, a);
     return hipGetLastError();
 }
 

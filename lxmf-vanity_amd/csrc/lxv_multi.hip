@@ -1,0 +1,533 @@
+// Host side of the batched search (lxv_multi_* in include/lxmf_vanity_hip.h):
+// admission into slots and the HBM state pool, per-step work planning, the
+// hipGraph-captured relaunch of the persistent kernel, and answer selection.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/lxmf_vanity_hip.h"
+#include "fe10.h"
+#include "fe8.h"
+#include "host_crypto.h"
+#include "lxv_kernels.h"
+#include "lxv_multi.h"
+
+using namespace lxv;
+
+namespace {
+
+constexpr uint64_t kStride = 1ull << 40;   // key distance between virtual lanes
+constexpr uint32_t kMaxHits = 1u << 20;
+constexpr uint32_t kNone = 0xffffffffu;
+
+thread_local std::string g_err = "no error";
+
+struct Search {
+    uint64_t id = 0, tag = 0, max_keys = 0;
+    uint8_t start[32];
+    SlotConst sc;
+    uint32_t vlanes = 0, slot = kNone, state_off = 0;
+    uint32_t next_block = 0, blocks_now = 0;
+    uint64_t keys = 0;
+    double submit_s = 0, admit_s = 0;
+    bool active = false;
+};
+
+// First-fit allocator of pool lanes (ranges are multiples of 64).
+struct Pool {
+    std::map<uint32_t, uint32_t> free;  // offset -> length
+    void init(uint32_t n) {
+        free.clear();
+        free[0] = n;
+    }
+    bool take(uint32_t len, uint32_t* off) {
+        for (auto it = free.begin(); it != free.end(); ++it) {
+            if (it->second >= len) {
+                *off = it->first;
+                const uint32_t rest = it->second - len, noff = it->first + len;
+                free.erase(it);
+                if (rest) free[noff] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void give(uint32_t off, uint32_t len) {
+        auto it = free.emplace(off, len).first;
+        auto nx = std::next(it);
+        if (nx != free.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free.erase(nx);
+        }
+        if (it != free.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free.erase(it);
+            }
+        }
+    }
+};
+
+uint32_t vlanes_for(int nibbles, uint32_t cap) {
+    // about 16^digits / 2^14 lanes: a search expects ~2^14 keys per lane, so
+    // the ladder that sets a lane up (~65 keys of work) costs under 1%.
+    const int log2_lanes = 4 * nibbles - 14;
+    uint64_t v = log2_lanes <= 6 ? 64 : (log2_lanes >= 18 ? (1u << 18) : (1ull << log2_lanes));
+    if (v > cap) v = cap;
+    if (v < 64) v = 64;
+    return (uint32_t)v;
+}
+
+uint32_t be_word(const uint8_t* b) {
+    return (uint32_t)b[0] << 24 | (uint32_t)b[1] << 16 | (uint32_t)b[2] << 8 | b[3];
+}
+
+}  // namespace
+
+struct lxv_multi {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err = "no error";
+    int layout = LXV_LAYOUT_8X32, batch = 256;
+    uint32_t rounds = 1, limbs = 8;
+    uint32_t phys_lanes = 0, pool_lanes = 0, max_slots = 1024, max_vlanes = 1u << 18;
+    bool use_graph = true;
+    // device
+    SlotConst* d_slots = nullptr;
+    SlotDyn* d_dyn = nullptr;
+    Unit* d_units = nullptr;
+    uint32_t* d_nunits = nullptr;
+    uint32_t* d_counters = nullptr;
+    uint32_t* d_pool = nullptr;
+    uint2* d_batch = nullptr;
+    MultiHit* d_hits = nullptr;
+    uint32_t* h_stop = nullptr;
+    uint32_t* d_stop = nullptr;
+    // pinned staging
+    Unit* h_units = nullptr;
+    uint32_t* h_nunits = nullptr;
+    uint32_t units_cap = 0;
+    MultiHit* h_hits = nullptr;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    MultiParams params;
+    // host bookkeeping
+    std::deque<uint64_t> queue;
+    std::map<uint64_t, Search> searches;
+    std::vector<uint64_t> slot_owner;
+    std::vector<uint32_t> free_slots;
+    std::deque<lxv_search_result> ready;
+    Pool pool;
+    uint64_t next_id = 1, keys = 0;
+    double busy_s = 0, last_ms = 0, rate = 6e9;
+    uint32_t done = 0, steps = 0;
+    std::chrono::steady_clock::time_point t0;
+    uint32_t q[8];
+
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+static int mfail(lxv_multi* m, int code, const std::string& msg) {
+    if (m)
+        m->err = msg;
+    else
+        g_err = msg;
+    return code;
+}
+
+#define MCHK(m, expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) return mfail(m, LXV_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static void multi_free(lxv_multi* m) {
+    if (!m) return;
+    (void)hipSetDevice(m->device);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    (void)hipFree(m->d_slots);
+    (void)hipFree(m->d_dyn);
+    (void)hipFree(m->d_units);
+    (void)hipFree(m->d_nunits);
+    (void)hipFree(m->d_counters);
+    (void)hipFree(m->d_pool);
+    (void)hipFree(m->d_batch);
+    (void)hipFree(m->d_hits);
+    if (m->h_stop) (void)hipHostFree(m->h_stop);
+    if (m->h_units) (void)hipHostFree(m->h_units);
+    if (m->h_nunits) (void)hipHostFree(m->h_nunits);
+    if (m->h_hits) (void)hipHostFree(m->h_hits);
+    if (m->ev0) (void)hipEventDestroy(m->ev0);
+    if (m->ev1) (void)hipEventDestroy(m->ev1);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+template <class F>
+static void q_limbs(const uint32_t q[8], uint32_t qp[10], uint32_t qm[10]) {
+    typedef typename F::fe fe;
+    const fe qf = F::from_words(q);
+    uint32_t a[F::kLimbs], b[F::kLimbs];
+    F::store(F::add(qf, F::one()), a);
+    F::store(F::sub(qf, F::one()), b);
+    for (int i = 0; i < 10; i++) {
+        qp[i] = i < F::kLimbs ? a[i] : 0;
+        qm[i] = i < F::kLimbs ? b[i] : 0;
+    }
+}
+
+extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
+    if (!out) return mfail(nullptr, LXV_E_ARG, "lxv_multi_open: out is NULL");
+    *out = nullptr;
+    lxv_multi_opts o{};
+    if (opts) {
+        if (opts->size < sizeof(lxv_multi_opts)) return mfail(nullptr, LXV_E_ARG, "lxv_multi_open: opts->size too small");
+        o = *opts;
+    }
+    auto* m = new (std::nothrow) lxv_multi();
+    if (!m) return mfail(nullptr, LXV_E_NOMEM, "lxv_multi_open: out of host memory");
+    m->t0 = std::chrono::steady_clock::now();
+    m->device = o.device < 0 ? 0 : o.device;
+    m->batch = o.batch ? (int)o.batch : 256;
+    m->layout = (int)o.layout;
+    m->use_graph = !o.no_graph;
+    if (o.max_searches) m->max_slots = o.max_searches;
+    if (o.max_vlanes) m->max_vlanes = std::max<uint32_t>(64, o.max_vlanes / 64 * 64);
+    if (!multi_supported(m->layout, m->batch)) {
+        delete m;
+        return mfail(nullptr, LXV_E_ARG, "lxv_multi_open: no batched kernel for this layout and batch");
+    }
+    int ndev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= m->device || hipSetDevice(m->device) != hipSuccess ||
+        hipGetDeviceProperties(&prop, m->device) != hipSuccess) {
+        delete m;
+        return mfail(nullptr, LXV_E_HIP, "no HIP device found (is this an MI355X host?)");
+    }
+    m->limbs = (uint32_t)limbs_of(m->layout);
+    m->phys_lanes = (uint32_t)prop.multiProcessorCount * 16u * 64u;
+    const uint64_t pool_bytes = o.pool_bytes ? o.pool_bytes : (2ull << 30);
+    const uint64_t lane_bytes = 4ull * m->limbs * sizeof(uint32_t);
+    m->pool_lanes = (uint32_t)std::min<uint64_t>(pool_bytes / lane_bytes / 64 * 64, 0xffffffc0ull);
+    if (m->pool_lanes < 64) {
+        delete m;
+        return mfail(nullptr, LXV_E_ARG, "lxv_multi_open: pool_bytes too small for one search");
+    }
+    if (m->max_vlanes > m->pool_lanes) m->max_vlanes = m->pool_lanes;
+    m->pool.init(m->pool_lanes);
+    m->slot_owner.assign(m->max_slots, 0);
+    for (uint32_t s = m->max_slots; s-- > 0;) m->free_slots.push_back(s);
+    uint8_t zero[32] = {0};
+    lxv_launch_args a{};
+    lxv_launch_args_for(zero, &a);
+    std::memcpy(m->q, a.q, sizeof m->q);
+
+#define MALLOC(ptr, bytes)                                                        \
+    do {                                                                          \
+        hipError_t e_ = hipMalloc((void**)&(ptr), (bytes));                        \
+        if (e_ != hipSuccess) {                                                   \
+            std::string msg = std::string("hipMalloc ") + #ptr + ": " + hipGetErrorString(e_); \
+            multi_free(m);                                                        \
+            return mfail(nullptr, LXV_E_HIP, msg);                                \
+        }                                                                         \
+    } while (0)
+    MALLOC(m->d_slots, (size_t)m->max_slots * sizeof(SlotConst));
+    MALLOC(m->d_dyn, (size_t)m->max_slots * sizeof(SlotDyn));
+    MALLOC(m->d_nunits, 64);
+    MALLOC(m->d_counters, 64);
+    MALLOC(m->d_pool, (size_t)m->pool_lanes * lane_bytes);
+    MALLOC(m->d_batch, (size_t)m->phys_lanes * m->batch * 2 * m->limbs * sizeof(uint32_t));
+    MALLOC(m->d_hits, (size_t)kMaxHits * sizeof(MultiHit));
+#undef MALLOC
+    hipError_t e = hipHostMalloc((void**)&m->h_stop, 64, hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&m->d_stop, m->h_stop, 0);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_nunits, 64, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_hits, (size_t)kMaxHits * sizeof(MultiHit), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&m->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&m->ev1);
+    if (e != hipSuccess) {
+        std::string msg = std::string("lxv_multi_open setup: ") + hipGetErrorString(e);
+        multi_free(m);
+        return mfail(nullptr, LXV_E_HIP, msg);
+    }
+    *(volatile uint32_t*)m->h_stop = 0;
+    std::memset(&m->params, 0, sizeof m->params);
+    if (m->layout == LXV_LAYOUT_8X32)
+        q_limbs<Field8>(m->q, m->params.qp, m->params.qm);
+    else
+        q_limbs<Field10>(m->q, m->params.qp, m->params.qm);
+    m->params.slots = m->d_slots;
+    m->params.dyn = m->d_dyn;
+    m->params.n_units = m->d_nunits;
+    m->params.counters = m->d_counters;
+    m->params.state = m->d_pool;
+    m->params.batch_mem = m->d_batch;
+    m->params.hits = m->d_hits;
+    m->params.stop = m->d_stop;
+    m->params.pool_lanes = m->pool_lanes;
+    m->params.phys_lanes = m->phys_lanes;
+    m->params.max_hits = kMaxHits;
+    *out = m;
+    return LXV_OK;
+}
+
+extern "C" int lxv_multi_submit(lxv_multi* m, const lxv_search_spec* spec, uint64_t* id) {
+    if (!m || !spec) return LXV_E_ARG;
+    if (!start_scalar_ok(spec->start)) return mfail(m, LXV_E_ARG, "lxv_multi_submit: not a valid start scalar");
+    char pfx[33];
+    std::memcpy(pfx, spec->prefix, 32);
+    pfx[32] = 0;
+    Prefix p;
+    std::string perr;
+    if (!Prefix::parse(pfx, &p, &perr)) return mfail(m, LXV_E_ARG, perr);
+    Search s;
+    s.id = m->next_id++;
+    s.tag = spec->tag;
+    s.max_keys = spec->max_keys;
+    std::memcpy(s.start, spec->start, 32);
+    s.vlanes = vlanes_for(p.nibbles, m->max_vlanes);
+    uint32_t ed[8], want[4], mask[4], nh[3];
+    for (int i = 0; i < 8; i++) ed[i] = be_word(spec->ed_pub + 4 * i);
+    p.words(want, mask);
+    uint8_t nhb[10];
+    name_hash(nhb);
+    nh[0] = be_word(nhb);
+    nh[1] = be_word(nhb + 4);
+    nh[2] = (uint32_t)nhb[8] << 24 | (uint32_t)nhb[9] << 16;
+    std::memset(&s.sc, 0, sizeof s.sc);
+    sha_search_init(s.sc.sha, ed, nh, want, mask);
+    // base = start / 8 - 1
+    uint8_t s8[32];
+    uint32_t carry = 0;
+    for (int i = 31; i >= 0; i--) {
+        s8[i] = (uint8_t)((spec->start[i] >> 3) | (carry << 5));
+        carry = spec->start[i] & 7;
+    }
+    for (int i = 0; i < 8; i++)
+        s.sc.base[i] = (uint32_t)s8[4 * i] | (uint32_t)s8[4 * i + 1] << 8 | (uint32_t)s8[4 * i + 2] << 16 |
+                       (uint32_t)s8[4 * i + 3] << 24;
+    for (int i = 0; i < 8; i++)
+        if (s.sc.base[i]-- != 0) break;
+    s.sc.stride = kStride;
+    s.sc.vlanes = s.vlanes;
+    s.sc.rounds = m->rounds;
+    s.submit_s = m->now();
+    m->searches[s.id] = s;
+    m->queue.push_back(s.id);
+    if (id) *id = s.id;
+    return LXV_OK;
+}
+
+static void finish(lxv_multi* m, Search& s, int status, uint64_t g, const uint32_t addr[4]) {
+    lxv_search_result r{};
+    r.id = s.id;
+    r.tag = s.tag;
+    r.g = g;
+    if (addr) std::memcpy(r.address, addr, 16);
+    r.keys = s.keys;
+    r.vlanes = s.vlanes;
+    r.status = status;
+    r.submit_s = s.submit_s;
+    r.admit_s = s.admit_s;
+    r.done_s = m->now();
+    m->ready.push_back(r);
+    if (s.active) {
+        m->pool.give(s.state_off, s.vlanes);
+        m->slot_owner[s.slot] = 0;
+        m->free_slots.push_back(s.slot);
+    }
+    m->done++;
+    m->searches.erase(s.id);
+}
+
+static int admit(lxv_multi* m) {
+    while (!m->queue.empty() && !m->free_slots.empty()) {
+        Search& s = m->searches[m->queue.front()];
+        uint32_t off;
+        if (!m->pool.take(s.vlanes, &off)) break;  // admission by device memory
+        m->queue.pop_front();
+        s.slot = m->free_slots.back();
+        m->free_slots.pop_back();
+        s.state_off = off;
+        s.sc.state_off = off;
+        s.sc.slot = s.slot;
+        s.active = true;
+        s.admit_s = m->now();
+        m->slot_owner[s.slot] = s.id;
+        SlotDyn dyn;
+        std::memset(&dyn, 0, sizeof dyn);
+        dyn.found_block = kNone;
+        MCHK(m, hipMemcpyAsync(m->d_slots + s.slot, &s.sc, sizeof s.sc, hipMemcpyHostToDevice, m->stream));
+        MCHK(m, hipMemcpyAsync(m->d_dyn + s.slot, &dyn, sizeof dyn, hipMemcpyHostToDevice, m->stream));
+        MCHK(m, launch_multi_init(m->layout, m->d_pool, m->pool_lanes, off, s.vlanes, s.sc.base, m->q, kStride,
+                                  m->stream));
+        // the staged structs are host stack/heap memory: wait before reuse
+        MCHK(m, hipStreamSynchronize(m->stream));
+    }
+    return LXV_OK;
+}
+
+static int launch_step(lxv_multi* m) {
+    if (m->use_graph) {
+        if (!m->gexec) {
+            MCHK(m, hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = launch_clear(m->d_counters, 16, m->stream);
+            if (e == hipSuccess) e = launch_multi(m->layout, m->batch, m->params, m->stream);
+            hipError_t e2 = hipStreamEndCapture(m->stream, &m->graph);
+            if (e != hipSuccess) return mfail(m, LXV_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            MCHK(m, e2);
+            MCHK(m, hipGraphInstantiate(&m->gexec, m->graph, nullptr, nullptr, 0));
+        }
+        MCHK(m, hipEventRecord(m->ev0, m->stream));
+        MCHK(m, hipGraphLaunch(m->gexec, m->stream));
+        MCHK(m, hipEventRecord(m->ev1, m->stream));
+    } else {
+        MCHK(m, launch_clear(m->d_counters, 16, m->stream));
+        MCHK(m, hipEventRecord(m->ev0, m->stream));
+        MCHK(m, launch_multi(m->layout, m->batch, m->params, m->stream));
+        MCHK(m, hipEventRecord(m->ev1, m->stream));
+    }
+    return LXV_OK;
+}
+
+extern "C" int lxv_multi_step(lxv_multi* m, uint32_t budget_ms, lxv_search_result* out, uint32_t cap,
+                              uint32_t* n_out) {
+    if (!m) return LXV_E_ARG;
+    MCHK(m, hipSetDevice(m->device));
+    int rc = admit(m);
+    if (rc) return rc;
+    // Plan: an equal share of ~budget_ms of work per active search, in whole
+    // blocks, capped at what one wave per virtual wave can walk in that time.
+    std::vector<Search*> act;
+    for (auto& kv : m->searches)
+        if (kv.second.active) act.push_back(&kv.second);
+    if (!act.empty()) {
+        const double budget_s = std::max(1u, budget_ms) / 1e3;
+        const double total = m->rate * budget_s;
+        const double share = total / act.size();
+        const double lane_keys = budget_s * m->rate / m->phys_lanes;  // keys one lane walks in the budget
+        size_t n_units = 0;
+        for (Search* s : act) {
+            const double block_keys = (double)s->vlanes * m->rounds * m->batch;
+            uint32_t nb = (uint32_t)std::max(1.0, std::floor(share / block_keys));
+            const uint32_t cap_blocks = (uint32_t)std::max(1.0, std::ceil(lane_keys / ((double)m->rounds * m->batch)));
+            nb = std::min(nb, cap_blocks);
+            s->blocks_now = nb;
+            n_units += s->vlanes / 64;
+        }
+        if (n_units > m->units_cap) {
+            if (m->h_units) (void)hipHostFree(m->h_units);
+            (void)hipFree(m->d_units);
+            m->h_units = nullptr;
+            m->d_units = nullptr;
+            m->units_cap = 0;
+            const size_t capn = std::max<size_t>(n_units, 4096);
+            MCHK(m, hipHostMalloc((void**)&m->h_units, capn * sizeof(Unit), hipHostMallocDefault));
+            MCHK(m, hipMalloc((void**)&m->d_units, capn * sizeof(Unit)));
+            m->units_cap = (uint32_t)capn;
+            m->params.units = m->d_units;
+            if (m->gexec) {
+                (void)hipGraphExecDestroy(m->gexec);
+                (void)hipGraphDestroy(m->graph);
+                m->gexec = nullptr;
+                m->graph = nullptr;
+            }
+        }
+        // Longest units first, so the tail of the launch is short units.
+        std::sort(act.begin(), act.end(), [](const Search* a, const Search* b) {
+            return a->blocks_now != b->blocks_now ? a->blocks_now > b->blocks_now : a->id < b->id;
+        });
+        size_t k = 0;
+        for (Search* s : act)
+            for (uint32_t w = 0; w < s->vlanes / 64; w++) m->h_units[k++] = Unit{s->slot, w, s->next_block, s->blocks_now};
+        *m->h_nunits = (uint32_t)k;
+        MCHK(m, hipMemcpyAsync(m->d_units, m->h_units, k * sizeof(Unit), hipMemcpyHostToDevice, m->stream));
+        MCHK(m, hipMemcpyAsync(m->d_nunits, m->h_nunits, sizeof(uint32_t), hipMemcpyHostToDevice, m->stream));
+        rc = launch_step(m);
+        if (rc) return rc;
+        MCHK(m, hipEventSynchronize(m->ev1));
+        float ms = 0;
+        MCHK(m, hipEventElapsedTime(&ms, m->ev0, m->ev1));
+        uint32_t counters[4];
+        MCHK(m, hipMemcpy(counters, m->d_counters, sizeof counters, hipMemcpyDeviceToHost));
+        const uint64_t kkeys = (uint64_t)counters[2] | (uint64_t)counters[3] << 32;
+        const uint32_t nh = counters[1];
+        if (nh > kMaxHits) return mfail(m, LXV_E_STATE, "lxv_multi_step: hit buffer overflow");
+        if (nh) MCHK(m, hipMemcpy(m->h_hits, m->d_hits, (size_t)nh * sizeof(MultiHit), hipMemcpyDeviceToHost));
+        m->steps++;
+        m->keys += kkeys;
+        m->busy_s += ms / 1e3;
+        m->last_ms = ms;
+        if (ms > 1.0 && kkeys > 0) m->rate = 0.5 * m->rate + 0.5 * (kkeys / (ms / 1e3));
+        // Answers: per slot, the lowest block with a hit, then the lowest (n, v).
+        std::map<uint32_t, const MultiHit*> best;
+        for (uint32_t i = 0; i < nh; i++) {
+            const MultiHit& h = m->h_hits[i];
+            auto it = best.find(h.slot);
+            if (it == best.end() || h.block < it->second->block ||
+                (h.block == it->second->block &&
+                 (h.n < it->second->n || (h.n == it->second->n && h.vlane < it->second->vlane))))
+                best[h.slot] = &h;
+        }
+        for (Search* s : act) {
+            auto it = best.find(s->slot);
+            if (it != best.end()) {
+                const MultiHit& h = *it->second;
+                s->keys += (uint64_t)(h.block - s->next_block + 1) * s->vlanes * m->rounds * m->batch;
+                finish(m, *s, 1, (uint64_t)h.vlane * kStride + h.n, h.address);
+            } else {
+                s->next_block += s->blocks_now;
+                s->keys += (uint64_t)s->blocks_now * s->vlanes * m->rounds * m->batch;
+                if (s->max_keys && s->keys >= s->max_keys) finish(m, *s, 0, 0, nullptr);
+            }
+        }
+    }
+    uint32_t n = 0;
+    while (out && n < cap && !m->ready.empty()) {
+        out[n++] = m->ready.front();
+        m->ready.pop_front();
+    }
+    if (n_out) *n_out = n;
+    return LXV_OK;
+}
+
+extern "C" int lxv_multi_cancel(lxv_multi* m, uint64_t id) {
+    if (!m) return LXV_E_ARG;
+    auto it = m->searches.find(id);
+    if (it == m->searches.end()) return mfail(m, LXV_E_ARG, "lxv_multi_cancel: no such search");
+    if (!it->second.active) m->queue.erase(std::find(m->queue.begin(), m->queue.end(), id));
+    finish(m, it->second, -1, 0, nullptr);
+    return LXV_OK;
+}
+
+extern "C" int lxv_multi_stats(lxv_multi* m, lxv_multi_stat* out) {
+    if (!m || !out) return LXV_E_ARG;
+    out->keys = m->keys;
+    out->busy_s = m->busy_s;
+    out->last_step_ms = m->last_ms;
+    uint32_t res = 0;
+    for (auto& kv : m->searches) res += kv.second.active ? 1 : 0;
+    out->resident = res;
+    out->queued = (uint32_t)m->queue.size();
+    out->done = m->done;
+    out->steps = m->steps;
+    return LXV_OK;
+}
+
+extern "C" void lxv_multi_close(lxv_multi* m) { multi_free(m); }
+
+extern "C" const char* lxv_multi_last_error(lxv_multi* m) { return m ? m->err.c_str() : g_err.c_str(); }

@@ -58,6 +58,30 @@ class LaunchStats(ctypes.Structure):
                 ("stopped", ctypes.c_uint32)]
 
 
+class MultiOpts(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("device", ctypes.c_int32), ("max_searches", ctypes.c_uint32),
+                ("batch", ctypes.c_uint32), ("layout", ctypes.c_uint32), ("no_graph", ctypes.c_uint32),
+                ("pool_bytes", ctypes.c_uint64), ("max_vlanes", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class SearchSpec(ctypes.Structure):
+    _fields_ = [("start", ctypes.c_uint8 * 32), ("ed_pub", ctypes.c_uint8 * 32), ("prefix", ctypes.c_char * 33),
+                ("reserved", ctypes.c_uint8 * 7), ("tag", ctypes.c_uint64), ("max_keys", ctypes.c_uint64)]
+
+
+class SearchResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("tag", ctypes.c_uint64), ("g", ctypes.c_uint64),
+                ("address", ctypes.c_uint32 * 4), ("keys", ctypes.c_uint64), ("vlanes", ctypes.c_uint32),
+                ("status", ctypes.c_int32), ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("done_s", ctypes.c_double)]
+
+
+class MultiStat(ctypes.Structure):
+    _fields_ = [("keys", ctypes.c_uint64), ("busy_s", ctypes.c_double), ("last_step_ms", ctypes.c_double),
+                ("resident", ctypes.c_uint32), ("queued", ctypes.c_uint32), ("done", ctypes.c_uint32),
+                ("steps", ctypes.c_uint32)]
+
+
 _lib = None
 
 
@@ -101,6 +125,15 @@ def lib():
     L.lxv_launch_args_for.restype = None
     L.lxv_seeded_search.argtypes = [ctypes.c_uint64, ctypes.c_int, vp, vp, vp]
     L.lxv_seeded_search.restype = None
+    L.lxv_multi_open.argtypes = [ctypes.POINTER(MultiOpts), ctypes.POINTER(vp)]
+    L.lxv_multi_submit.argtypes = [vp, ctypes.POINTER(SearchSpec), ctypes.POINTER(ctypes.c_uint64)]
+    L.lxv_multi_step.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(SearchResult), ctypes.c_uint32, u32p]
+    L.lxv_multi_cancel.argtypes = [vp, ctypes.c_uint64]
+    L.lxv_multi_stats.argtypes = [vp, ctypes.POINTER(MultiStat)]
+    L.lxv_multi_close.argtypes = [vp]
+    L.lxv_multi_close.restype = None
+    L.lxv_multi_last_error.argtypes = [vp]
+    L.lxv_multi_last_error.restype = ctypes.c_char_p
     L.lxv_debug_fe.argtypes = [ctypes.c_int, ctypes.c_int, u32p, u32p, u32p, ctypes.c_uint32, ctypes.c_int]
     L.lxv_debug_address.argtypes = [u32p, u32p, u32p, u32p, ctypes.c_uint32, ctypes.c_int]
     _lib = L
@@ -325,6 +358,77 @@ class GpuSearch:
 
     def close(self):
         self.backend.close()
+
+
+# ---- batched searches (many searches in one persistent kernel)
+
+STRIDE_MULTI = 1 << 40   # key distance between a search's virtual lanes
+
+
+class Multi:
+    """lxv_multi_*: submit searches, step the device, collect answers.
+
+    A result is a dict: id, tag, g (key = start + 8 g), address (16 bytes),
+    keys, vlanes, status (1 found, 0 gave up, -1 cancelled), submit_s,
+    admit_s, done_s (seconds since open)."""
+
+    def __init__(self, max_searches=0, batch=256, layout=LAYOUT_8X32, no_graph=False, pool_bytes=0,
+                 max_vlanes=0, device=0):
+        o = MultiOpts(ctypes.sizeof(MultiOpts), device, max_searches, batch, layout, 1 if no_graph else 0,
+                      pool_bytes, max_vlanes, 0)
+        h = ctypes.c_void_p()
+        if lib().lxv_multi_open(ctypes.byref(o), ctypes.byref(h)) != LXV_OK:
+            raise GpuUnavailable(lib().lxv_multi_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != LXV_OK:
+            raise RuntimeError("%s: %s" % (what, lib().lxv_multi_last_error(self.h).decode()))
+
+    def submit(self, start, ed_pub, prefix, tag=0, max_keys=0):
+        s = SearchSpec()
+        s.start[:] = list(start)
+        s.ed_pub[:] = list(ed_pub)
+        s.prefix = prefix.encode()
+        s.tag, s.max_keys = tag, max_keys
+        i = ctypes.c_uint64(0)
+        self._check(lib().lxv_multi_submit(self.h, ctypes.byref(s), ctypes.byref(i)), "submit")
+        return i.value
+
+    def step(self, budget_ms=200, cap=4096):
+        out = (SearchResult * cap)()
+        n = ctypes.c_uint32(0)
+        self._check(lib().lxv_multi_step(self.h, budget_ms, out, cap, ctypes.byref(n)), "step")
+        return [{"id": r.id, "tag": r.tag, "g": r.g, "address": words_to_bytes_be(r.address), "keys": r.keys,
+                 "vlanes": r.vlanes, "status": r.status, "submit_s": r.submit_s, "admit_s": r.admit_s,
+                 "done_s": r.done_s} for r in out[:n.value]]
+
+    def cancel(self, sid):
+        self._check(lib().lxv_multi_cancel(self.h, sid), "cancel")
+
+    def stats(self):
+        s = MultiStat()
+        self._check(lib().lxv_multi_stats(self.h, ctypes.byref(s)), "stats")
+        return {k: getattr(s, k) for k, _ in MultiStat._fields_}
+
+    def run_all(self, budget_ms=200, timeout_s=600.0):
+        """Steps until every submitted search is finished; returns results."""
+        res, t0 = [], time.time()
+        while True:
+            res += self.step(budget_ms)
+            st = self.stats()
+            if st["resident"] == 0 and st["queued"] == 0:
+                return res + self.step(budget_ms)
+            if time.time() - t0 > timeout_s:
+                raise TimeoutError("searches still running after %.0f s" % timeout_s)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lxv_multi_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def debug_fe(op, a_words, b_words, on_gpu, layout=LAYOUT_8X32):
