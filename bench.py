@@ -13,6 +13,17 @@ v_mad_u64_u32 issue peak, and the CPU oracle (the C restatement of the
 reference's CPU walk) timed on this host's cores on a bounded sample.
 
     python bench.py [--steps K] [--warmup W] [--budget-log2 36] [--no-cpu]
+
+Other workloads (one JSON line each, same keys where they apply):
+
+    python bench.py --workload batched [--wall-s 60]
+        256 concurrent seeded searches (4-10 hex digits) in the persistent
+        kernel; value = aggregate keys/s, compared with the plain kernel on
+        one search measured in the same process.
+    python bench.py --workload serving [--duration-s 60] [--policy short-first|fifo] [--one-by-one]
+        Poisson arrivals (20/s, 5-9 hex digits); p50/p99 time-to-hit from
+        arrival, queue depth, unfinished requests; --one-by-one also replays
+        the same arrivals through the plain kernel one search at a time.
 """
 
 import argparse
@@ -83,8 +94,68 @@ def load_traffic():
         return None
 
 
+def plain_rate(batch=256, waves_per_cu=16, budget_log2=33):
+    """keys/s of the plain kernel on one search (workload seed 2, 8 hex
+    digits), kernel time only: the yardstick for the many-search workloads."""
+    start, ed_seed, prefix = lxv.seeded_search(2, 8)
+    s = lxv.GpuSearch(start=start, ed_seed=ed_seed, batch=batch, waves_per_cu=waves_per_cu)
+    try:
+        b = s.backend
+        iters = max(1, (1 << budget_log2) // (b.threads * b.batch))
+        a = lxv.launch_args_for(s.ed_pub, prefix, iters=iters)
+        b.launch(lxv.launch_args_for(s.ed_pub, prefix, iters=1))
+        keys, ms = 0, 0.0
+        for _ in range(2):
+            b.launch(a)
+            st = b.stats()
+            keys += st.keys
+            ms += st.kernel_ms
+        return keys / (ms / 1e3)
+    finally:
+        s.close()
+
+
+def many_search_main(args):
+    import workloads
+    log = lambda msg: print(msg, file=sys.stderr, flush=True)  # noqa: E731
+    base = plain_rate()
+    log("plain kernel, one search (B=256, 16 waves/CU): %.3e keys/s" % base)
+    common = {"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+              "dtype": "u32 (exact integer limbs)",
+              "data": "synthetic (seeded start scalars, Ed25519 keys and prefixes)", "plain_keys_per_s": base}
+    if args.workload == "batched":
+        r = workloads.batched(wall_s=args.wall_s, progress=log)
+        out = {"metric": "aggregate keys/s, 256 concurrent searches", "value": r["keys_per_s"], "unit": "keys/s",
+               "steps": r["steps"], "warmup": 0, "ms_per_step": 1e3 * r["wall_s"] / max(1, r["steps"]),
+               "config": {"workload": "batched: 256 seeded searches, 4-10 hex digits "
+                                      "(P=.10/.15/.20/.20/.15/.12/.08), %.0f s wall" % args.wall_s,
+                          "keys_per_inversion": 256, "layout": "8x32"},
+               "vs_plain": r["keys_per_s"] / base, "kernel_vs_plain": (r["kernel_keys_per_s"] or 0) / base}
+        out.update(common)
+        out["result"] = r
+    else:
+        r = workloads.serving(duration_s=args.duration_s, policy=args.policy, progress=log)
+        out = {"metric": "p50/p99 time-to-hit, mixed prefixes (serving)", "value": r["keys_per_s"],
+               "unit": "keys/s", "steps": r["steps"], "warmup": 0,
+               "ms_per_step": 1e3 * r["wall_s"] / max(1, r["steps"]),
+               "p50_s": r["p50_s"], "p99_s": r["p99_s"], "unfinished": r["unfinished"],
+               "config": {"workload": "serving: Poisson 20/s for %.0f s, 5-9 hex digits (P=.30/.30/.20/.15/.05), "
+                                      "admission %s" % (args.duration_s, args.policy),
+                          "keys_per_inversion": 256, "layout": "8x32"}}
+        out.update(common)
+        out["result"] = r
+        if args.one_by_one:
+            out["one_by_one"] = workloads.one_by_one(duration_s=args.duration_s, progress=log)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=("plain", "batched", "serving"), default="plain")
+    ap.add_argument("--wall-s", type=float, default=60.0)
+    ap.add_argument("--duration-s", type=float, default=60.0)
+    ap.add_argument("--policy", choices=("short-first", "fifo"), default="short-first")
+    ap.add_argument("--one-by-one", action="store_true")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
@@ -100,6 +171,9 @@ def main():
         print("bench.py measures one GPU: this build walks on a single MI355X and has no multi-GPU path "
               "(a search shards by disjoint lane ranges with no exchange; see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "plain":
+        many_search_main(args)
+        return
 
     start, ed_seed, prefix = lxv.seeded_search(1, 7)
     ed_pub = lxv.ed25519_public(ed_seed)

@@ -197,7 +197,8 @@ void lxv_seeded_search(uint64_t seed, int nibbles, uint8_t start[32], uint8_t ed
  * reference (one search per process); the serving loop is built on these.
  *
  * A search owns V virtual lanes (64 to 262144, by prefix length: about
- * 16^digits / 2^14) whose walk state stays resident in an HBM pool; virtual
+ * 16^digits / 2^lane_keys_log2, by default 2^14 expected keys per lane)
+ * whose walk state stays resident in an HBM pool; virtual
  * lane v walks keys start + 8 (v x 2^40 + n) and all of them advance
  * together in blocks of B keys. Each step relaunches the kernel (a captured
  * hipGraph) over a list of work units (search, 64 virtual lanes, blocks);
@@ -205,8 +206,10 @@ void lxv_seeded_search(uint64_t seed, int nibbles, uint8_t start[32], uint8_t ed
  * are done move on to unfinished ones. A search's answer is its first hit in
  * walk order: the lowest block with a hit, then the lowest (n, v) in it. Every
  * block up to that one is walked completely before the answer is reported, so
- * it does not depend on what else runs. Searches are admitted in submission
- * order as slots and pool memory allow.
+ * it does not depend on what else runs. Queued searches are admitted at
+ * the start of a step, by (priority, submission order), as slots and pool
+ * memory allow; one that does not fit yet does not hold back smaller ones
+ * queued after it.
  */
 
 typedef struct lxv_multi lxv_multi;
@@ -220,14 +223,18 @@ typedef struct lxv_multi_opts {
     uint32_t no_graph;      /* 1: launch eagerly instead of replaying a hipGraph */
     uint64_t pool_bytes;    /* device bytes for resident walk state; 0 = 2 GiB */
     uint32_t max_vlanes;    /* cap on a search's virtual lanes; 0 = 262144 */
-    uint32_t reserved;
+    uint32_t lane_keys_log2;/* expected keys per virtual lane, log2 (8..20); 0 = 14.
+                               Smaller: more lanes per search, so a short search
+                               finishes in fewer sequential blocks, at the cost of
+                               one X25519 ladder per extra lane */
 } lxv_multi_opts;
 
 typedef struct lxv_search_spec {
     uint8_t start[32];      /* first private key of the search (as random_start_scalar) */
     uint8_t ed_pub[32];     /* Ed25519 public key of the identity */
     char prefix[33];        /* 1-32 hex digits, NUL-terminated */
-    uint8_t reserved[7];
+    uint8_t reserved[3];
+    uint32_t priority;      /* admission order: lower first, then submission order */
     uint64_t tag;           /* caller's cookie, returned with the result */
     uint64_t max_keys;      /* give up after about this many keys; 0 = no limit */
 } lxv_search_spec;

@@ -30,10 +30,19 @@ thread_local std::string g_err = "no error";
 
 struct Search {
     uint64_t id = 0, tag = 0, max_keys = 0;
+    uint32_t priority = 0;
     uint8_t start[32];
     SlotConst sc;
     uint32_t vlanes = 0, slot = kNone, state_off = 0;
-    uint32_t next_block = 0, blocks_now = 0;
+    // Progress: blocks [0, next_block) are walked completely, and virtual
+    // waves [0, wave_cursor) of block next_block as well.
+    uint32_t next_block = 0, wave_cursor = 0;
+    // This step's units: virtual waves [w0, w1) of blocks [next_block, + nblocks).
+    uint32_t w0 = 0, w1 = 0, nblocks = 0;
+    // Best hit so far in walk order (block, n, v); final once its block is whole.
+    bool hit = false;
+    uint32_t hit_block = 0, hit_vlane = 0, hit_addr[4] = {0, 0, 0, 0};
+    uint64_t hit_n = 0;
     uint64_t keys = 0;
     double submit_s = 0, admit_s = 0;
     bool active = false;
@@ -75,10 +84,11 @@ struct Pool {
     }
 };
 
-uint32_t vlanes_for(int nibbles, uint32_t cap) {
-    // about 16^digits / 2^14 lanes: a search expects ~2^14 keys per lane, so
-    // the ladder that sets a lane up (~65 keys of work) costs under 1%.
-    const int log2_lanes = 4 * nibbles - 14;
+uint32_t vlanes_for(int nibbles, uint32_t cap, uint32_t lane_keys_log2) {
+    // about 16^digits / 2^lane_keys_log2 lanes: at the default 2^14 expected
+    // keys per lane, the ladder that sets a lane up (a few hundred keys' worth
+    // of field work) costs about 1% of the search.
+    const int log2_lanes = 4 * nibbles - (int)lane_keys_log2;
     uint64_t v = log2_lanes <= 6 ? 64 : (log2_lanes >= 18 ? (1u << 18) : (1ull << log2_lanes));
     if (v > cap) v = cap;
     if (v < 64) v = 64;
@@ -98,7 +108,7 @@ struct lxv_multi {
     std::string err = "no error";
     int layout = LXV_LAYOUT_8X32, batch = 256;
     uint32_t rounds = 1, limbs = 8;
-    uint32_t phys_lanes = 0, pool_lanes = 0, max_slots = 1024, max_vlanes = 1u << 18;
+    uint32_t phys_lanes = 0, pool_lanes = 0, max_slots = 1024, max_vlanes = 1u << 18, lane_keys_log2 = 14;
     bool use_graph = true;
     // device
     SlotConst* d_slots = nullptr;
@@ -206,6 +216,13 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
     m->use_graph = !o.no_graph;
     if (o.max_searches) m->max_slots = o.max_searches;
     if (o.max_vlanes) m->max_vlanes = std::max<uint32_t>(64, o.max_vlanes / 64 * 64);
+    if (o.lane_keys_log2) {
+        if (o.lane_keys_log2 < 8 || o.lane_keys_log2 > 20) {
+            delete m;
+            return mfail(nullptr, LXV_E_ARG, "lxv_multi_open: lane_keys_log2 must be 8..20");
+        }
+        m->lane_keys_log2 = o.lane_keys_log2;
+    }
     if (!multi_supported(m->layout, m->batch)) {
         delete m;
         return mfail(nullptr, LXV_E_ARG, "lxv_multi_open: no batched kernel for this layout and batch");
@@ -298,8 +315,9 @@ extern "C" int lxv_multi_submit(lxv_multi* m, const lxv_search_spec* spec, uint6
     s.id = m->next_id++;
     s.tag = spec->tag;
     s.max_keys = spec->max_keys;
+    s.priority = spec->priority;
     std::memcpy(s.start, spec->start, 32);
-    s.vlanes = vlanes_for(p.nibbles, m->max_vlanes);
+    s.vlanes = vlanes_for(p.nibbles, m->max_vlanes, m->lane_keys_log2);
     uint32_t ed[8], want[4], mask[4], nh[3];
     for (int i = 0; i < 8; i++) ed[i] = be_word(spec->ed_pub + 4 * i);
     p.words(want, mask);
@@ -354,12 +372,22 @@ static void finish(lxv_multi* m, Search& s, int status, uint64_t g, const uint32
     m->searches.erase(s.id);
 }
 
+// Admits queued searches by (priority, submission order), each as soon as a
+// slot and a pool range of its size are free; one that does not fit yet does
+// not hold back smaller ones behind it.
 static int admit(lxv_multi* m) {
-    while (!m->queue.empty() && !m->free_slots.empty()) {
-        Search& s = m->searches[m->queue.front()];
+    if (m->queue.empty() || m->free_slots.empty()) return LXV_OK;
+    std::stable_sort(m->queue.begin(), m->queue.end(), [m](uint64_t a, uint64_t b) {
+        return m->searches.at(a).priority < m->searches.at(b).priority;
+    });
+    for (auto qi = m->queue.begin(); qi != m->queue.end() && !m->free_slots.empty();) {
+        Search& s = m->searches.at(*qi);
         uint32_t off;
-        if (!m->pool.take(s.vlanes, &off)) break;  // admission by device memory
-        m->queue.pop_front();
+        if (!m->pool.take(s.vlanes, &off)) {  // admission by device memory
+            ++qi;
+            continue;
+        }
+        qi = m->queue.erase(qi);
         s.slot = m->free_slots.back();
         m->free_slots.pop_back();
         s.state_off = off;
@@ -375,7 +403,7 @@ static int admit(lxv_multi* m) {
         MCHK(m, hipMemcpyAsync(m->d_dyn + s.slot, &dyn, sizeof dyn, hipMemcpyHostToDevice, m->stream));
         MCHK(m, launch_multi_init(m->layout, m->d_pool, m->pool_lanes, off, s.vlanes, s.sc.base, m->q, kStride,
                                   m->stream));
-        // the staged structs are host stack/heap memory: wait before reuse
+        // the staged structs are host memory: wait before they are reused
         MCHK(m, hipStreamSynchronize(m->stream));
     }
     return LXV_OK;
@@ -410,24 +438,60 @@ extern "C" int lxv_multi_step(lxv_multi* m, uint32_t budget_ms, lxv_search_resul
     MCHK(m, hipSetDevice(m->device));
     int rc = admit(m);
     if (rc) return rc;
-    // Plan: an equal share of ~budget_ms of work per active search, in whole
-    // blocks, capped at what one wave per virtual wave can walk in that time.
+    // Plan about budget_ms of work. Each active search may use at most its
+    // lanes' worth of the chip for the budget. Priorities are served in
+    // ascending order; searches of one priority share what is left equally
+    // (water-filling), so a search may get nothing this step and keep its
+    // state for later. A search whose share is at least a block walks whole
+    // blocks; otherwise it walks that many virtual waves of its current block,
+    // and the block completes over the next steps.
     std::vector<Search*> act;
     for (auto& kv : m->searches)
         if (kv.second.active) act.push_back(&kv.second);
     if (!act.empty()) {
         const double budget_s = std::max(1u, budget_ms) / 1e3;
-        const double total = m->rate * budget_s;
-        const double share = total / act.size();
-        const double lane_keys = budget_s * m->rate / m->phys_lanes;  // keys one lane walks in the budget
+        const double wave_keys = 64.0 * m->rounds * m->batch;          // one virtual wave, one block
+        const double lane_keys = budget_s * m->rate / m->phys_lanes;  // one lane, busy for the budget
+        std::vector<std::pair<double, Search*>> want;
+        for (Search* s : act)
+            want.push_back({(double)std::min(s->vlanes, m->phys_lanes) * std::max(lane_keys, (double)m->rounds * m->batch), s});
+        std::sort(want.begin(), want.end(), [](const std::pair<double, Search*>& a, const std::pair<double, Search*>& b) {
+            if (a.second->priority != b.second->priority) return a.second->priority < b.second->priority;
+            return a.first != b.first ? a.first < b.first : a.second->id < b.second->id;
+        });
+        double left = m->rate * budget_s;
         size_t n_units = 0;
-        for (Search* s : act) {
-            const double block_keys = (double)s->vlanes * m->rounds * m->batch;
-            uint32_t nb = (uint32_t)std::max(1.0, std::floor(share / block_keys));
-            const uint32_t cap_blocks = (uint32_t)std::max(1.0, std::ceil(lane_keys / ((double)m->rounds * m->batch)));
-            nb = std::min(nb, cap_blocks);
-            s->blocks_now = nb;
-            n_units += s->vlanes / 64;
+        for (size_t i = 0; i < want.size(); i++) {
+            Search* s = want[i].second;
+            size_t tier_end = i;
+            while (tier_end < want.size() && want[tier_end].second->priority == s->priority) tier_end++;
+            const double share = std::min(want[i].first, left / (double)(tier_end - i));
+            left -= share;
+            const uint32_t vwaves = s->vlanes / 64;
+            const double block_keys = wave_keys * vwaves;
+            if (share < 0.5 * wave_keys) {
+                s->w0 = s->w1 = s->wave_cursor;  // nothing this step
+                s->nblocks = 0;
+            } else if (s->wave_cursor == 0 && share >= block_keys && !s->hit) {
+                s->w0 = 0;
+                s->w1 = vwaves;
+                s->nblocks = (uint32_t)std::floor(share / block_keys);
+            } else {
+                const uint32_t k = (uint32_t)std::max(1.0, std::round(share / wave_keys));
+                s->w0 = s->wave_cursor;
+                s->w1 = std::min(vwaves, s->wave_cursor + k);
+                s->nblocks = 1;
+            }
+            n_units += s->w1 - s->w0;
+        }
+        if (n_units == 0) {
+            // shares below one unit each: the first search in service order
+            // still walks one virtual wave, so every step makes progress
+            Search* s = want[0].second;
+            s->w0 = s->wave_cursor;
+            s->w1 = s->wave_cursor + 1;
+            s->nblocks = 1;
+            n_units = 1;
         }
         if (n_units > m->units_cap) {
             if (m->h_units) (void)hipHostFree(m->h_units);
@@ -449,11 +513,11 @@ extern "C" int lxv_multi_step(lxv_multi* m, uint32_t budget_ms, lxv_search_resul
         }
         // Longest units first, so the tail of the launch is short units.
         std::sort(act.begin(), act.end(), [](const Search* a, const Search* b) {
-            return a->blocks_now != b->blocks_now ? a->blocks_now > b->blocks_now : a->id < b->id;
+            return a->nblocks != b->nblocks ? a->nblocks > b->nblocks : a->id < b->id;
         });
         size_t k = 0;
         for (Search* s : act)
-            for (uint32_t w = 0; w < s->vlanes / 64; w++) m->h_units[k++] = Unit{s->slot, w, s->next_block, s->blocks_now};
+            for (uint32_t w = s->w0; w < s->w1; w++) m->h_units[k++] = Unit{s->slot, w, s->next_block, s->nblocks};
         *m->h_nunits = (uint32_t)k;
         MCHK(m, hipMemcpyAsync(m->d_units, m->h_units, k * sizeof(Unit), hipMemcpyHostToDevice, m->stream));
         MCHK(m, hipMemcpyAsync(m->d_nunits, m->h_nunits, sizeof(uint32_t), hipMemcpyHostToDevice, m->stream));
@@ -473,27 +537,43 @@ extern "C" int lxv_multi_step(lxv_multi* m, uint32_t budget_ms, lxv_search_resul
         m->busy_s += ms / 1e3;
         m->last_ms = ms;
         if (ms > 1.0 && kkeys > 0) m->rate = 0.5 * m->rate + 0.5 * (kkeys / (ms / 1e3));
-        // Answers: per slot, the lowest block with a hit, then the lowest (n, v).
-        std::map<uint32_t, const MultiHit*> best;
+        // Per search, keep the lowest hit in walk order: block, then n, then v.
+        std::vector<Search*> by_slot(m->max_slots, nullptr);
+        for (Search* s : act) by_slot[s->slot] = s;
         for (uint32_t i = 0; i < nh; i++) {
             const MultiHit& h = m->h_hits[i];
-            auto it = best.find(h.slot);
-            if (it == best.end() || h.block < it->second->block ||
-                (h.block == it->second->block &&
-                 (h.n < it->second->n || (h.n == it->second->n && h.vlane < it->second->vlane))))
-                best[h.slot] = &h;
-        }
-        for (Search* s : act) {
-            auto it = best.find(s->slot);
-            if (it != best.end()) {
-                const MultiHit& h = *it->second;
-                s->keys += (uint64_t)(h.block - s->next_block + 1) * s->vlanes * m->rounds * m->batch;
-                finish(m, *s, 1, (uint64_t)h.vlane * kStride + h.n, h.address);
-            } else {
-                s->next_block += s->blocks_now;
-                s->keys += (uint64_t)s->blocks_now * s->vlanes * m->rounds * m->batch;
-                if (s->max_keys && s->keys >= s->max_keys) finish(m, *s, 0, 0, nullptr);
+            Search* s = h.slot < m->max_slots ? by_slot[h.slot] : nullptr;
+            if (!s) return mfail(m, LXV_E_STATE, "lxv_multi_step: hit for a slot with no search");
+            if (!s->hit || h.block < s->hit_block ||
+                (h.block == s->hit_block && (h.n < s->hit_n || (h.n == s->hit_n && h.vlane < s->hit_vlane)))) {
+                s->hit = true;
+                s->hit_block = h.block;
+                s->hit_n = h.n;
+                s->hit_vlane = h.vlane;
+                std::memcpy(s->hit_addr, h.address, sizeof s->hit_addr);
             }
+        }
+        const double block_of = (double)m->rounds * m->batch;  // keys per lane per block
+        for (Search* s : act) {
+            const uint32_t vwaves = s->vlanes / 64;
+            if (s->w0 == 0 && s->w1 == vwaves) {
+                s->next_block += s->nblocks;  // whole blocks (a hit stops units only past its block)
+            } else {
+                s->wave_cursor = s->w1;
+                if (s->wave_cursor == vwaves) {
+                    s->wave_cursor = 0;
+                    s->next_block += 1;
+                }
+            }
+            if (s->hit && s->next_block > s->hit_block) {
+                // every block up to the hit's is walked: the answer is final
+                s->keys = (uint64_t)(s->hit_block + 1) * s->vlanes * (uint64_t)block_of;
+                finish(m, *s, 1, (uint64_t)s->hit_vlane * kStride + s->hit_n, s->hit_addr);
+                continue;
+            }
+            s->keys = (uint64_t)s->next_block * s->vlanes * (uint64_t)block_of +
+                      (uint64_t)s->wave_cursor * 64 * (uint64_t)block_of;
+            if (s->max_keys && s->keys >= s->max_keys) finish(m, *s, 0, 0, nullptr);
         }
     }
     uint32_t n = 0;

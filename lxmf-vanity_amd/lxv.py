@@ -61,12 +61,13 @@ class LaunchStats(ctypes.Structure):
 class MultiOpts(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("device", ctypes.c_int32), ("max_searches", ctypes.c_uint32),
                 ("batch", ctypes.c_uint32), ("layout", ctypes.c_uint32), ("no_graph", ctypes.c_uint32),
-                ("pool_bytes", ctypes.c_uint64), ("max_vlanes", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("pool_bytes", ctypes.c_uint64), ("max_vlanes", ctypes.c_uint32), ("lane_keys_log2", ctypes.c_uint32)]
 
 
 class SearchSpec(ctypes.Structure):
     _fields_ = [("start", ctypes.c_uint8 * 32), ("ed_pub", ctypes.c_uint8 * 32), ("prefix", ctypes.c_char * 33),
-                ("reserved", ctypes.c_uint8 * 7), ("tag", ctypes.c_uint64), ("max_keys", ctypes.c_uint64)]
+                ("reserved", ctypes.c_uint8 * 3), ("priority", ctypes.c_uint32), ("tag", ctypes.c_uint64),
+                ("max_keys", ctypes.c_uint64)]
 
 
 class SearchResult(ctypes.Structure):
@@ -373,9 +374,9 @@ class Multi:
     admit_s, done_s (seconds since open)."""
 
     def __init__(self, max_searches=0, batch=256, layout=LAYOUT_8X32, no_graph=False, pool_bytes=0,
-                 max_vlanes=0, device=0):
+                 max_vlanes=0, lane_keys_log2=0, device=0):
         o = MultiOpts(ctypes.sizeof(MultiOpts), device, max_searches, batch, layout, 1 if no_graph else 0,
-                      pool_bytes, max_vlanes, 0)
+                      pool_bytes, max_vlanes, lane_keys_log2)
         h = ctypes.c_void_p()
         if lib().lxv_multi_open(ctypes.byref(o), ctypes.byref(h)) != LXV_OK:
             raise GpuUnavailable(lib().lxv_multi_last_error(None).decode())
@@ -385,12 +386,12 @@ class Multi:
         if rc != LXV_OK:
             raise RuntimeError("%s: %s" % (what, lib().lxv_multi_last_error(self.h).decode()))
 
-    def submit(self, start, ed_pub, prefix, tag=0, max_keys=0):
+    def submit(self, start, ed_pub, prefix, tag=0, max_keys=0, priority=0):
         s = SearchSpec()
         s.start[:] = list(start)
         s.ed_pub[:] = list(ed_pub)
         s.prefix = prefix.encode()
-        s.tag, s.max_keys = tag, max_keys
+        s.tag, s.max_keys, s.priority = tag, max_keys, priority
         i = ctypes.c_uint64(0)
         self._check(lib().lxv_multi_submit(self.h, ctypes.byref(s), ctypes.byref(i)), "submit")
         return i.value

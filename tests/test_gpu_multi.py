@@ -25,10 +25,11 @@ def workload(seeds, digits):
     return out
 
 
-def run(searches, budget_ms=50, **opts):
+def run(searches, budget_ms=50, priorities=None, **opts):
     m = lxv.Multi(**opts)
     try:
-        ids = [m.submit(st, ed, pf, tag=i) for i, (st, ed, pf) in enumerate(searches)]
+        prio = priorities or [0] * len(searches)
+        ids = [m.submit(st, ed, pf, tag=i, priority=prio[i]) for i, (st, ed, pf) in enumerate(searches)]
         print("searches", len(ids), opts, flush=True)
         res = {r["tag"]: r for r in m.run_all(budget_ms=budget_ms)}
         assert sorted(res) == list(range(len(ids)))
@@ -64,9 +65,31 @@ def test_same_answer_alone_and_in_a_mix():
     others = workload(range(300, 320), [2, 3, 4, 5] * 5)
     alone = [run([t])[0][0] for t in targets]
     mixed, _ = run(others[:10] + targets + others[10:])
+    # priorities change who is admitted and served first, never an answer
+    prio = [i % 4 for i in range(10)] + [5, 5, 5] + [i % 3 for i in range(10)]
+    ranked, _ = run(others[:10] + targets + others[10:], budget_ms=5, priorities=prio)
     for i, a in enumerate(alone):
-        m = mixed[10 + i]
-        assert (m["g"], m["address"]) == (a["g"], a["address"])
+        for res in (mixed, ranked):
+            m = res[10 + i]
+            assert (m["g"], m["address"]) == (a["g"], a["address"])
+
+
+def test_blocks_split_across_steps_keep_the_answer():
+    # 6 digits at 2^8 expected keys per lane: 65536 virtual lanes, so a block
+    # is 2^24 keys, far above a 1 ms step's share: each block is walked a few
+    # virtual waves per step. The answer must match whole-block steps and the
+    # oracle, and `keys` must count every block up to the answer's.
+    searches = workload(range(700, 704), [6] * 4)
+    split, st = run(searches, budget_ms=1, lane_keys_log2=8)
+    whole, _ = run(searches, budget_ms=2000, lane_keys_log2=8)
+    assert st["steps"] > 8
+    for i, (start, ed_pub, prefix) in enumerate(searches):
+        r = split[i]
+        assert r["vlanes"] == 65536 and r["status"] == 1
+        assert (r["g"], r["address"]) == (whole[i]["g"], whole[i]["address"])
+        block = (r["g"] % STRIDE) // B
+        assert r["keys"] == (block + 1) * r["vlanes"] * B
+        assert (r["g"], r["address"]) == oracle_first_hit(start, ed_pub, prefix, r["vlanes"], block)
 
 
 def test_graph_replay_matches_eager_launches():
