@@ -124,7 +124,10 @@ def many_search_main(args):
               "dtype": "u32 (exact integer limbs)",
               "data": "synthetic (seeded start scalars, Ed25519 keys and prefixes)", "plain_keys_per_s": base}
     if args.workload == "batched":
-        r = workloads.batched(wall_s=args.wall_s, progress=log)
+        kw = {"lane_keys_log2": args.lane_keys_log2} if args.lane_keys_log2 else {}
+        if args.budget_ms:
+            kw["budget_ms"] = args.budget_ms
+        r = workloads.batched(wall_s=args.wall_s, progress=log, **kw)
         out = {"metric": "aggregate keys/s, 256 concurrent searches", "value": r["keys_per_s"], "unit": "keys/s",
                "steps": r["steps"], "warmup": 0, "ms_per_step": 1e3 * r["wall_s"] / max(1, r["steps"]),
                "config": {"workload": "batched: 256 seeded searches, 4-10 hex digits "
@@ -134,13 +137,17 @@ def many_search_main(args):
         out.update(common)
         out["result"] = r
     else:
-        r = workloads.serving(duration_s=args.duration_s, policy=args.policy, progress=log)
+        kw = {"lane_keys_log2": args.lane_keys_log2} if args.lane_keys_log2 else {}
+        if args.budget_ms:
+            kw["budget_ms"] = args.budget_ms
+        r = workloads.serving(duration_s=args.duration_s, policy=args.policy, progress=log, **kw)
         out = {"metric": "p50/p99 time-to-hit, mixed prefixes (serving)", "value": r["keys_per_s"],
                "unit": "keys/s", "steps": r["steps"], "warmup": 0,
                "ms_per_step": 1e3 * r["wall_s"] / max(1, r["steps"]),
                "p50_s": r["p50_s"], "p99_s": r["p99_s"], "unfinished": r["unfinished"],
                "config": {"workload": "serving: Poisson 20/s for %.0f s, 5-9 hex digits (P=.30/.30/.20/.15/.05), "
-                                      "admission %s" % (args.duration_s, args.policy),
+                                      "admission %s, %d ms steps, 2^%d keys per lane" % (args.duration_s, args.policy, r["budget_ms"],
+                                                                          r["lane_keys_log2"]),
                           "keys_per_inversion": 256, "layout": "8x32"}}
         out.update(common)
         out["result"] = r
@@ -156,6 +163,9 @@ def main():
     ap.add_argument("--duration-s", type=float, default=60.0)
     ap.add_argument("--policy", choices=("short-first", "fifo"), default="short-first")
     ap.add_argument("--one-by-one", action="store_true")
+    ap.add_argument("--lane-keys-log2", type=int, default=0,
+                    help="serving/batched: expected keys per virtual lane, log2 (0: the workload's default)")
+    ap.add_argument("--budget-ms", type=int, default=0, help="serving/batched: step budget (0: the workload's default)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
@@ -213,12 +223,32 @@ def main():
     roofline = {"bound": "valu-imad", "achieved": achieved / 1e12, "peak": (mad_peak or 0) / 1e12,
                 "unit": "T mad/s", "frac": (achieved / mad_peak) if mad_peak else None,
                 "traffic": None, "mads_per_key": work}
+    # HBM: the algorithmic bytes per key are the batch arrays (a_i and Z_i,
+    # written in the forward pass and read back in the backward pass) plus
+    # the lane state read and written once per launch.
+    limbs = 10 if args.layout == lxv.LAYOUT_10X26 else 8
+    fe_bytes = 4 * limbs
+    alg_bytes_per_key = 2 * 2 * fe_bytes + 2 * 4 * fe_bytes / (iters * b.batch)
+    roofline["hbm"] = {"algorithmic_bytes_per_key": alg_bytes_per_key,
+                       "achieved_GBps": value * alg_bytes_per_key / 1e9, "peak_GBps": 8000.0,
+                       "frac": value * alg_bytes_per_key / 8e12}
     pmc = load_traffic()
-    if pmc and "hbm_bytes_per_key" in pmc:
+    wpc = args.waves_per_cu or 16
+    kname = "walk_kernel<lxv::%s, %d, %d>" % ("Field10" if args.layout == lxv.LAYOUT_10X26 else "Field8", b.batch,
+                                             4 if wpc >= 16 else (3 if wpc >= 12 else (2 if wpc >= 8 else 1)))
+    if pmc and kname in pmc.get("kernel", "") and "hbm_bytes_per_key" in pmc:
         # bytes per launch of this run = measured bytes per key x keys per launch
         roofline["traffic"] = pmc["hbm_bytes_per_key"] * per_iter * iters
-        roofline["traffic_source"] = "profiles/walk_pmc.json (FETCH_SIZE/WRITE_SIZE, copy-calibrated)"
-        roofline["valu_insts_per_key"] = pmc.get("valu_insts_per_key")
+        roofline["traffic_source"] = "profiles/walk_pmc.json (FETCH_SIZE/WRITE_SIZE, copy-calibrated, same kernel)"
+        v = pmc.get("valu_insts_per_key")
+        if v and clock and cus:
+            # VALU issue: wave64 instructions at one per SIMD per 4 clocks,
+            # i.e. CUs x 64 lane-instructions per clock at the measured clock
+            peak_issue = cus * 64 * clock
+            roofline["valu_issue"] = {"valu_insts_per_key": v, "achieved": value * v / 1e12,
+                                      "peak": peak_issue / 1e12, "unit": "T lane-instr/s",
+                                      "frac": value * v / peak_issue,
+                                      "busy_pct_measured": pmc.get("valu_busy_pct")}
 
     out = {
         "metric": METRIC, "value": value, "unit": "keys/s", "n_gpus": 1, "steps": args.steps,

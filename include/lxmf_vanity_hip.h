@@ -55,7 +55,7 @@ typedef struct lxv_open_opts {
     int32_t device;        /* HIP device ordinal; negative means 0 */
     uint32_t threads;      /* walk lanes; 0 = CUs x waves_per_cu x 64 */
     uint32_t batch;        /* keys per batched inversion (B); 0 = 64 */
-    uint32_t waves_per_cu; /* resident waves per CU the grid is sized for; 0 = 8 */
+    uint32_t waves_per_cu; /* resident waves per CU the grid is sized for; 0 = 16 */
     uint32_t layout;       /* LXV_LAYOUT_*; 0 (8x32) is the default */
 } lxv_open_opts;
 
@@ -137,7 +137,9 @@ int lxv_poll(lxv_ctx* ctx, int wait, lxv_hit* hits, uint32_t cap, uint32_t* n_hi
 /* Asks a running launch to end early: every lane finishes the batch it is
  * in, stores its state and exits, so the walk can resume with the next
  * launch. No counterpart in the reference's trait; it is the "stop" of the
- * host seam and what lets a long launch give way to a found hit. */
+ * host seam and what lets a long launch give way to a found hit. It only
+ * sets a flag in host memory, so unlike every other call it may come from
+ * another thread while the context's own thread waits in lxv_poll. */
 int lxv_stop(lxv_ctx* ctx);
 
 /* XOR of the LXMF addresses of every key walked since the last init, as 4

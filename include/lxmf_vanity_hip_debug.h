@@ -54,6 +54,12 @@ int lxv_debug_copy(int device, uint64_t bytes, double* ms);
 int lxv_debug_kernel_attrs(int layout, uint32_t batch, uint32_t waves_per_cu, uint32_t* vgprs, uint32_t* sgprs,
                            uint32_t* scratch_bytes, uint32_t* waves_per_simd);
 
+/* The command-line tool's CPU key walk (KeyWalk, src/main.rs:229-290, on
+ * the host field arithmetic): the public keys u(start + 8n) for
+ * n < batches x 256, as 32-byte little-endian encodings into `out`
+ * (batches x 256 x 32 bytes). CPU only. */
+int lxv_debug_cpu_walk(const uint8_t start[32], uint32_t batches, uint8_t* out);
+
 #ifdef __cplusplus
 }
 #endif

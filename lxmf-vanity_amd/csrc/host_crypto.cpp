@@ -440,15 +440,21 @@ void name_hash(uint8_t out[10]) {
     std::memcpy(out, h, 10);
 }
 
-void lxmf_address(const uint8_t x_pub[32], const uint8_t ed_pub[32], uint8_t out[16]) {
+void lxmf_address_nh(const uint8_t nh[10], const uint8_t x_pub[32], const uint8_t ed_pub[32], uint8_t out[16]) {
     uint8_t pk[64], ih[32], material[26], a[32];
     std::memcpy(pk, x_pub, 32);
     std::memcpy(pk + 32, ed_pub, 32);
     sha256(pk, 64, ih);
-    name_hash(material);
+    std::memcpy(material, nh, 10);
     std::memcpy(material + 10, ih, 16);
     sha256(material, 26, a);
     std::memcpy(out, a, 16);
+}
+
+void lxmf_address(const uint8_t x_pub[32], const uint8_t ed_pub[32], uint8_t out[16]) {
+    uint8_t nh[10];
+    name_hash(nh);
+    lxmf_address_nh(nh, x_pub, ed_pub, out);
 }
 
 void address_of_identity(const uint8_t identity[64], uint8_t out[16]) {
@@ -541,6 +547,68 @@ std::string seeded_prefix(uint64_t seed, int nibbles) {
     if (nibbles < 0) nibbles = 0;
     if (nibbles > 64) nibbles = 64;
     return h.substr(0, (size_t)nibbles);
+}
+
+// ---------------------------------------------------------------------------
+// The CPU key walk.
+
+static inline F f_load(const uint64_t v[4]) { return F{{v[0], v[1], v[2], v[3]}}; }
+static inline void f_store(uint64_t v[4], const F& a) {
+    for (int i = 0; i < 4; i++) v[i] = a.v[i];
+}
+
+KeyWalk::KeyWalk(const uint8_t start[32]) {
+    std::memcpy(start_, start, 32);
+    uint8_t prev[32], u[32], q[32], eight[32] = {8};
+    previous_scalar(start, prev);
+    x25519_base(prev, u);
+    f_store(prev_x_, f_frombytes(u));
+    x25519_base(start, u);
+    f_store(cur_x_, f_frombytes(u));
+    f_store(prev_z_, f_from_u64(1));
+    f_store(cur_z_, f_from_u64(1));
+    mont_mul(eight, BASE_U, q);  // u(8B): the walk's fixed step
+    const F qf = f_frombytes(q), one = f_from_u64(1);
+    f_store(q_plus_, f_add(qf, one));
+    f_store(q_minus_, f_sub(qf, one));
+}
+
+uint64_t KeyWalk::next_batch(uint8_t (*out)[32], size_t len) {
+    if (len > kBatch) len = kBatch;
+    const uint64_t first = next_n_;
+    F xs[kBatch], zs[kBatch], pre[kBatch];
+    F xp = f_load(prev_x_), zp = f_load(prev_z_), x = f_load(cur_x_), z = f_load(cur_z_);
+    const F qp = f_load(q_plus_), qm = f_load(q_minus_);
+    for (size_t i = 0; i < len; i++) {
+        xs[i] = x;
+        zs[i] = z;
+        // P(n+1) = P(n) + Q with P(n) - Q = P(n-1), Q = 8B affine
+        const F t1 = f_mul(f_sub(x, z), qp), t2 = f_mul(f_add(x, z), qm);
+        const F xn = f_mul(zp, f_sq(f_add(t1, t2)));
+        const F zn = f_mul(xp, f_sq(f_sub(t1, t2)));
+        xp = x;
+        zp = z;
+        x = xn;
+        z = zn;
+    }
+    // one inversion for the batch (Montgomery's trick)
+    F acc = f_from_u64(1);
+    for (size_t i = 0; i < len; i++) {
+        pre[i] = acc;
+        acc = f_mul(acc, zs[i]);
+    }
+    F inv = f_inv(acc);
+    for (size_t i = len; i-- > 0;) {
+        const F zinv = f_mul(inv, pre[i]);
+        inv = f_mul(inv, zs[i]);
+        f_tobytes(out[i], f_mul(xs[i], zinv));
+    }
+    f_store(prev_x_, xp);
+    f_store(prev_z_, zp);
+    f_store(cur_x_, x);
+    f_store(cur_z_, z);
+    next_n_ += len;
+    return first;
 }
 
 std::string hex(const uint8_t* b, size_t n) {

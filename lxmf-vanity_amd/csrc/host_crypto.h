@@ -35,6 +35,8 @@ bool random_bytes(uint8_t* out, size_t n);
 // Addresses: src/main.rs:174-197.
 void name_hash(uint8_t out[10]);
 void lxmf_address(const uint8_t x_pub[32], const uint8_t ed_pub[32], uint8_t out[16]);
+// The same with the name hash supplied, for loops over many keys.
+void lxmf_address_nh(const uint8_t nh[10], const uint8_t x_pub[32], const uint8_t ed_pub[32], uint8_t out[16]);
 void address_of_identity(const uint8_t identity[64], uint8_t out[16]);
 
 // A parsed hex prefix: src/main.rs:199-225.
@@ -55,5 +57,24 @@ void seeded_ed_seed(uint64_t seed, uint8_t out[32]);
 std::string seeded_prefix(uint64_t seed, int nibbles);
 
 std::string hex(const uint8_t* b, size_t n);
+
+// The CPU key walk of the search command's CPU workers: u(k + 8n) for n = 0,
+// 1, 2, ... by x-only differential addition with one inversion per batch
+// (KeyWalk, src/main.rs:229-290). Host arithmetic is the 4 x 64-bit field
+// above; this is product code, independent of the test oracle.
+class KeyWalk {
+  public:
+    explicit KeyWalk(const uint8_t start[32]);
+    // Fills out[0..len) (len <= kBatch) with the next public keys; returns
+    // the n of the first.
+    uint64_t next_batch(uint8_t (*out)[32], size_t len);
+    const uint8_t* start() const { return start_; }
+    static constexpr size_t kBatch = 256;
+
+  private:
+    uint8_t start_[32];
+    uint64_t next_n_ = 0;
+    uint64_t prev_x_[4], prev_z_[4], cur_x_[4], cur_z_[4], q_plus_[4], q_minus_[4];
+};
 
 }  // namespace lxv

@@ -43,10 +43,13 @@ hipError_t launch_init_ladder(int layout, uint32_t* state, uint64_t* ns, uint32_
 hipError_t launch_clear(uint32_t* words, uint32_t n, hipStream_t s);
 
 struct MultiParams;
+struct MultiInitEntry;
 bool multi_supported(int layout, int batch);
 hipError_t launch_multi(int layout, int batch, const MultiParams& p, hipStream_t s);
-hipError_t launch_multi_init(int layout, uint32_t* pool, uint32_t pool_lanes, uint32_t state_off, uint32_t vlanes,
-                             const uint32_t base[8], const uint32_t q[8], uint64_t stride, hipStream_t s);
+// Sets up the virtual lanes of n admitted searches (entries in device memory)
+// in one launch; max_vlanes bounds every entry's lane count.
+hipError_t launch_multi_init(int layout, uint32_t* pool, uint32_t pool_lanes, const uint32_t q[8], uint64_t stride,
+                             const MultiInitEntry* entries, uint32_t n, uint32_t max_vlanes, hipStream_t s);
 hipError_t launch_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n,
                            hipStream_t s);
 // lanes must be a multiple of 256; 8 mads per lane per iteration.

@@ -107,18 +107,18 @@ __global__ __launch_bounds__(64, 4) void multi_walk_kernel(const MultiParams p) 
 }
 
 struct MultiInitArgs {
-    uint32_t base[8];
     uint32_t q[8];
     uint64_t stride;
     uint32_t pool_lanes;
-    uint32_t state_off;
-    uint32_t vlanes;
+    const MultiInitEntry* entries;
 };
 
+// Block (x, y): lanes 64 x .. 64 x + 63 of admitted search y.
 template <class F>
 __global__ __launch_bounds__(64) void multi_init_kernel(uint32_t* pool, const MultiInitArgs a) {
+    const MultiInitEntry& e = a.entries[blockIdx.y];
     const uint32_t v = blockIdx.x * 64u + threadIdx.x;
-    if (v < a.vlanes) multi_init_lane<F>(pool, a.pool_lanes, a.state_off, a.base, a.stride, a.q, v);
+    if (v < e.vlanes) multi_init_lane<F>(pool, a.pool_lanes, e.state_off, e.base, a.stride, a.q, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -143,6 +143,8 @@ static const WalkEntry kWalks[] = {
     LXV_W3(Field8, LXV_LAYOUT_8X32, 32),    LXV_W3(Field8, LXV_LAYOUT_8X32, 64),
     LXV_W3(Field8, LXV_LAYOUT_8X32, 128),   LXV_W3(Field8, LXV_LAYOUT_8X32, 256),
 #undef LXV_W3
+    // 3 waves per SIMD (168 VGPRs): between the spill-free 2 and the spilling 4
+    LXV_WALK_ENTRY(Field8, LXV_LAYOUT_8X32, 64, 3), LXV_WALK_ENTRY(Field8, LXV_LAYOUT_8X32, 256, 3),
 };
 
 bool walk_supported(int layout, int batch, int waves) {
@@ -226,18 +228,16 @@ hipError_t launch_multi(int layout, int batch, const MultiParams& p, hipStream_t
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_multi_init(int layout, uint32_t* pool, uint32_t pool_lanes, uint32_t state_off, uint32_t vlanes,
-                             const uint32_t base[8], const uint32_t q[8], uint64_t stride, hipStream_t s) {
+hipError_t launch_multi_init(int layout, uint32_t* pool, uint32_t pool_lanes, const uint32_t q[8], uint64_t stride,
+                             const MultiInitEntry* entries, uint32_t n, uint32_t max_vlanes, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (n > 65535) return hipErrorInvalidValue;
     MultiInitArgs a;
-    for (int i = 0; i < 8; i++) {
-        a.base[i] = base[i];
-        a.q[i] = q[i];
-    }
+    for (int i = 0; i < 8; i++) a.q[i] = q[i];
     a.stride = stride;
     a.pool_lanes = pool_lanes;
-    a.state_off = state_off;
-    a.vlanes = vlanes;
-    const dim3 grid((vlanes + 63) / 64), block(64);
+    a.entries = entries;
+    const dim3 grid((max_vlanes + 63) / 64, n), block(64);
     if (layout == LXV_LAYOUT_8X32)
         hipLaunchKernelGGL(multi_init_kernel<Field8>, grid, block, 0, s, pool, a);
     else

@@ -36,6 +36,15 @@ struct SlotDyn {
     uint32_t pad[15];      // one 64-byte line per slot
 };
 
+// One admitted search's virtual lanes, set up by the ladder in one batched
+// launch per step (block row = entry).
+struct MultiInitEntry {
+    uint32_t base[8];    // start / 8 - 1, little-endian words
+    uint32_t state_off;  // first pool lane
+    uint32_t vlanes;
+    uint32_t pad[2];
+};
+
 struct Unit {
     uint32_t slot;
     uint32_t vwave;

@@ -121,7 +121,13 @@ struct lxv_multi {
     MultiHit* d_hits = nullptr;
     uint32_t* h_stop = nullptr;
     uint32_t* d_stop = nullptr;
-    // pinned staging
+    // pinned staging: slot tables mirrored on the host, admission entries
+    SlotConst* h_slots = nullptr;
+    SlotDyn* h_dyn = nullptr;
+    MultiInitEntry* h_init = nullptr;
+    MultiInitEntry* d_init = nullptr;
+    hipEvent_t ev_admit = nullptr;
+    bool admit_pending = false;
     Unit* h_units = nullptr;
     uint32_t* h_nunits = nullptr;
     uint32_t units_cap = 0;
@@ -176,6 +182,11 @@ static void multi_free(lxv_multi* m) {
     (void)hipFree(m->d_pool);
     (void)hipFree(m->d_batch);
     (void)hipFree(m->d_hits);
+    (void)hipFree(m->d_init);
+    if (m->h_slots) (void)hipHostFree(m->h_slots);
+    if (m->h_dyn) (void)hipHostFree(m->h_dyn);
+    if (m->h_init) (void)hipHostFree(m->h_init);
+    if (m->ev_admit) (void)hipEventDestroy(m->ev_admit);
     if (m->h_stop) (void)hipHostFree(m->h_stop);
     if (m->h_units) (void)hipHostFree(m->h_units);
     if (m->h_nunits) (void)hipHostFree(m->h_nunits);
@@ -268,10 +279,15 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
     MALLOC(m->d_pool, (size_t)m->pool_lanes * lane_bytes);
     MALLOC(m->d_batch, (size_t)m->phys_lanes * m->batch * 2 * m->limbs * sizeof(uint32_t));
     MALLOC(m->d_hits, (size_t)kMaxHits * sizeof(MultiHit));
+    MALLOC(m->d_init, (size_t)m->max_slots * sizeof(MultiInitEntry));
 #undef MALLOC
     hipError_t e = hipHostMalloc((void**)&m->h_stop, 64, hipHostMallocMapped | hipHostMallocCoherent);
     if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&m->d_stop, m->h_stop, 0);
     if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_nunits, 64, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_slots, (size_t)m->max_slots * sizeof(SlotConst), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_dyn, (size_t)m->max_slots * sizeof(SlotDyn), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_init, (size_t)m->max_slots * sizeof(MultiInitEntry), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev_admit, hipEventDisableTiming);
     if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_hits, (size_t)kMaxHits * sizeof(MultiHit), hipHostMallocDefault);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&m->ev0);
@@ -374,12 +390,20 @@ static void finish(lxv_multi* m, Search& s, int status, uint64_t g, const uint32
 
 // Admits queued searches by (priority, submission order), each as soon as a
 // slot and a pool range of its size are free; one that does not fit yet does
-// not hold back smaller ones behind it.
+// not hold back smaller ones behind it. The admitted searches' slot tables
+// are copied from pinned host mirrors and their lanes set up by one ladder
+// launch, all enqueued ahead of the step's walk on the same stream.
 static int admit(lxv_multi* m) {
     if (m->queue.empty() || m->free_slots.empty()) return LXV_OK;
+    if (m->admit_pending) {
+        // the staging entries of the last admission are read by its copy
+        MCHK(m, hipEventSynchronize(m->ev_admit));
+        m->admit_pending = false;
+    }
     std::stable_sort(m->queue.begin(), m->queue.end(), [m](uint64_t a, uint64_t b) {
         return m->searches.at(a).priority < m->searches.at(b).priority;
     });
+    uint32_t n = 0, max_vl = 0;
     for (auto qi = m->queue.begin(); qi != m->queue.end() && !m->free_slots.empty();) {
         Search& s = m->searches.at(*qi);
         uint32_t off;
@@ -396,15 +420,29 @@ static int admit(lxv_multi* m) {
         s.active = true;
         s.admit_s = m->now();
         m->slot_owner[s.slot] = s.id;
-        SlotDyn dyn;
-        std::memset(&dyn, 0, sizeof dyn);
-        dyn.found_block = kNone;
-        MCHK(m, hipMemcpyAsync(m->d_slots + s.slot, &s.sc, sizeof s.sc, hipMemcpyHostToDevice, m->stream));
-        MCHK(m, hipMemcpyAsync(m->d_dyn + s.slot, &dyn, sizeof dyn, hipMemcpyHostToDevice, m->stream));
-        MCHK(m, launch_multi_init(m->layout, m->d_pool, m->pool_lanes, off, s.vlanes, s.sc.base, m->q, kStride,
+        // a slot's host mirror is rewritten only here, after the step that
+        // last read it has been waited for
+        m->h_slots[s.slot] = s.sc;
+        std::memset(&m->h_dyn[s.slot], 0, sizeof(SlotDyn));
+        m->h_dyn[s.slot].found_block = kNone;
+        MCHK(m, hipMemcpyAsync(m->d_slots + s.slot, m->h_slots + s.slot, sizeof(SlotConst), hipMemcpyHostToDevice,
+                               m->stream));
+        MCHK(m, hipMemcpyAsync(m->d_dyn + s.slot, m->h_dyn + s.slot, sizeof(SlotDyn), hipMemcpyHostToDevice,
+                               m->stream));
+        MultiInitEntry& e = m->h_init[n++];
+        std::memcpy(e.base, s.sc.base, sizeof e.base);
+        e.state_off = off;
+        e.vlanes = s.vlanes;
+        e.pad[0] = e.pad[1] = 0;
+        max_vl = std::max(max_vl, s.vlanes);
+    }
+    if (n) {
+        MCHK(m, hipMemcpyAsync(m->d_init, m->h_init, (size_t)n * sizeof(MultiInitEntry), hipMemcpyHostToDevice,
+                               m->stream));
+        MCHK(m, launch_multi_init(m->layout, m->d_pool, m->pool_lanes, m->q, kStride, m->d_init, n, max_vl,
                                   m->stream));
-        // the staged structs are host memory: wait before they are reused
-        MCHK(m, hipStreamSynchronize(m->stream));
+        MCHK(m, hipEventRecord(m->ev_admit, m->stream));
+        m->admit_pending = true;
     }
     return LXV_OK;
 }

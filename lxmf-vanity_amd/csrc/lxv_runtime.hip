@@ -122,7 +122,7 @@ extern "C" int lxv_open(const lxv_open_opts* opts, lxv_ctx** out) {
     }
     c->batch = o.batch ? o.batch : 64;
     uint32_t wpc = o.waves_per_cu ? o.waves_per_cu : 16;
-    c->waves = wpc >= 16 ? 4 : (wpc >= 8 ? 2 : 1);
+    c->waves = wpc >= 16 ? 4 : (wpc >= 12 ? 3 : (wpc >= 8 ? 2 : 1));
     if (!walk_supported((int)c->layout, (int)c->batch, (int)c->waves)) {
         delete c;
         return fail(nullptr, LXV_E_ARG,
@@ -458,6 +458,14 @@ extern "C" int lxv_debug_fe(int layout, int op, const uint32_t* a, const uint32_
     return e == hipSuccess ? LXV_OK : LXV_E_HIP;
 }
 
+extern "C" int lxv_debug_cpu_walk(const uint8_t start[32], uint32_t batches, uint8_t* out) {
+    if (!start || !out || !start_scalar_ok(start)) return LXV_E_ARG;
+    KeyWalk w(start);
+    for (uint32_t b = 0; b < batches; b++)
+        w.next_batch(reinterpret_cast<uint8_t(*)[32]>(out + (size_t)b * KeyWalk::kBatch * 32), KeyWalk::kBatch);
+    return LXV_OK;
+}
+
 extern "C" int lxv_debug_address(const uint32_t ed[8], const uint32_t nh[3], const uint32_t* u, uint32_t* out,
                                  uint32_t n, int on_gpu) {
     if (!ed || !nh || !u || !out) return LXV_E_ARG;
@@ -537,7 +545,7 @@ extern "C" int lxv_debug_copy(int device, uint64_t bytes, double* ms) {
 
 extern "C" int lxv_debug_kernel_attrs(int layout, uint32_t batch, uint32_t waves_per_cu, uint32_t* vgprs,
                                       uint32_t* sgprs, uint32_t* scratch_bytes, uint32_t* waves_per_simd) {
-    const int w = waves_per_cu >= 16 ? 4 : (waves_per_cu >= 8 ? 2 : 1);
+    const int w = waves_per_cu >= 16 ? 4 : (waves_per_cu >= 12 ? 3 : (waves_per_cu >= 8 ? 2 : 1));
     if (!walk_supported(layout, (int)batch, w)) return LXV_E_ARG;
     hipFuncAttributes a;
     if (walk_attributes(layout, (int)batch, w, &a) != hipSuccess) return LXV_E_HIP;

@@ -151,7 +151,7 @@ def _latency_report(reqs, end):
             "per_length": table}
 
 
-def serving(rate=20.0, duration_s=60.0, policy="short-first", budget_ms=50, seed=4, lane_keys_log2=12,
+def serving(rate=20.0, duration_s=60.0, policy="short-first", budget_ms=50, seed=4, lane_keys_log2=10,
             pool_bytes=8 << 30, progress=print):
     """The serving loop. Requests are admitted into lxv_multi in `policy`
     order: "short-first" (fewest expected keys, 16^digits, then arrival) or
@@ -203,6 +203,7 @@ def serving(rate=20.0, duration_s=60.0, policy="short-first", budget_ms=50, seed
         m.close()
     out = _latency_report(reqs, end)
     out.update({"policy": policy, "rate_per_s": rate, "duration_s": duration_s, "wall_s": end,
+                "budget_ms": budget_ms, "lane_keys_log2": lane_keys_log2,
                 "keys": st["keys"], "busy_s": st["busy_s"], "steps": st["steps"],
                 "keys_per_s": st["keys"] / end, "rechecked": out["finished"],
                 "queue_depth": [{"t_s": t, "queued": w, "resident": k} for t, w, k in depth[::5]]})

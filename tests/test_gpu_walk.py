@@ -5,6 +5,8 @@ XOR checksum of every walked address must match the oracle bit for bit on the
 same seeded inputs. Lane t walks keys start + 8 (t x stride + n).
 """
 
+import json
+import os
 import random
 import time
 
@@ -191,6 +193,49 @@ def test_stop_ends_a_long_launch_and_the_walk_resumes(small_backend):
             assert lxv.words_to_bytes_le(dump[(t * 64 + i) * 8:(t * 64 + i) * 8 + 8]) == orc.x25519(k)
     for t, n, addr in hits + h2:
         assert addr == orc.lxmf_address(orc.x25519(orc.step_scalar(start, t * STRIDE + n)), ed_pub)
+
+
+@pytest.mark.parametrize("layout", [lxv.LAYOUT_8X32, lxv.LAYOUT_10X26], ids=["8x32", "10x26"])
+def test_config0_workload_matches_golden(layout):
+    # The small workload at full size: 2^24 consecutive keys of workload
+    # seed 0 as 65536 lanes x 256 keys (lane t walks g in [256 t, 256 t + 256)),
+    # every hit of the 4-hex prefix and the XOR of all 2^24 addresses against
+    # the committed fixture.
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config0.json")) as f:
+        fx = json.load(f)
+    start, ed_pub = bytes.fromhex(fx["start"]), bytes.fromhex(fx["ed_pub"])
+    b = lxv.Backend(threads=65536, batch=256, layout=layout)
+    try:
+        b.init_seeded(start, 256)
+        hits, _ = b.launch(lxv.launch_args_for(ed_pub, fx["prefix"], iters=1, max_hits=1 << 12))
+        assert b.stats().keys == fx["keys"]
+        got = sorted((t * 256 + n, a.hex()) for t, n, a in hits)
+        assert got == [tuple(h) for h in fx["hits"]]
+        assert b.checksum().hex() == fx["xor"]
+    finally:
+        b.close()
+
+
+def test_config1_subrange_checksum_matches_oracle():
+    # The headline workload's shape (workload seed 1, 7-hex prefix, B = 64,
+    # every lane of the chip at 2^40 keys apart): a 2^28-key sub-range, lanes
+    # 0..L-1 x n < 2^28 / L, walked on the GPU and by the oracle; every hit
+    # and the XOR of all 2^28 addresses must agree.
+    start, _, ed_pub, prefix = seeded(1, 7)
+    b = lxv.Backend(batch=64)
+    try:
+        per_lane = (1 << 28) // b.threads
+        assert per_lane * b.threads == 1 << 28 and per_lane % 64 == 0
+        b.init_seeded(start, lxv.GpuSearch.LANE_STRIDE)
+        hits, _ = b.launch(lxv.launch_args_for(ed_pub, prefix, iters=per_lane // 64, max_hits=1 << 12))
+        gpu_xor = b.checksum()
+        ranges = [(t * lxv.GpuSearch.LANE_STRIDE, per_lane) for t in range(b.threads)]
+        print("lanes", b.threads, "keys per lane", per_lane, "hits", len(hits), flush=True)
+        want, xor = orc.scan_ranges(start, ed_pub, ranges, prefix, cap=1 << 12)
+        assert sorted((t * lxv.GpuSearch.LANE_STRIDE + n, a) for t, n, a in hits) == want
+        assert gpu_xor == xor
+    finally:
+        b.close()
 
 
 def test_self_test_and_run():

@@ -174,3 +174,21 @@ def test_device_address_hash_on_host():
     out = lxv.debug_address(list(a.ed), list(a.name_hash), U, on_gpu=False)
     for i, u in enumerate(us):
         assert lxv.words_to_bytes_be(out[4 * i:4 * i + 4]) == orc.lxmf_address(u, ed_pub)
+
+
+def test_cli_cpu_walk_matches_x25519_across_batches():
+    # the command-line tool's CPU worker walk (src/main.rs:527-539's check)
+    import ctypes
+    L = lxv.lib()
+    L.lxv_debug_cpu_walk.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_void_p]
+    start, _, _ = lxv.seeded_search(11, 0)
+    out = ctypes.create_string_buffer(3 * 256 * 32)
+    assert L.lxv_debug_cpu_walk(start, 3, out) == 0
+    keys = out.raw
+    for n in list(range(0, 768, 37)) + [255, 256, 511, 512, 767]:
+        assert keys[32 * n:32 * n + 32] == orc.x25519(orc.step_scalar(start, n)), n
+    w = orc.Walk(start)
+    walk = []
+    for _ in range(3):
+        walk += w.next_batch()[1]
+    assert [keys[32 * n:32 * n + 32] for n in range(768)] == walk

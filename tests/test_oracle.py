@@ -250,6 +250,21 @@ def test_scan_matches_golden(seed):
         assert [(g, a.hex()) for g, a in hits] == [tuple(h) for h in fx["hits"]]
 
 
+def test_config0_workload_matches_golden():
+    # 2^24 consecutive keys of workload seed 0, 4-hex prefix: every hit and
+    # the XOR of all addresses (tests/golden/config0.json, cross-checked
+    # against the Python walk on its first 2^16 keys when it was written)
+    with open(os.path.join(GOLDEN, "config0.json")) as f:
+        fx = json.load(f)
+    start = bytes.fromhex(fx["start"])
+    assert start == tb.seeded_start(0)
+    ed_pub = bytes.fromhex(fx["ed_pub"])
+    hits, xor = orc.scan_ranges(start, ed_pub, [(0, fx["keys"])], fx["prefix"], cap=1 << 12)
+    assert xor.hex() == fx["xor"]
+    assert [(g, a.hex()) for g, a in hits] == [tuple(h) for h in fx["hits"]]
+    assert 200 < len(hits) < 320
+
+
 def test_scan_detects_a_planted_mistake():
     # The comparison above is sensitive: one key off, or a different Ed25519
     # half, gives a different checksum and hit list.
