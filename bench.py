@@ -235,7 +235,7 @@ def main():
     pmc = load_traffic()
     wpc = args.waves_per_cu or 16
     kname = "walk_kernel<lxv::%s, %d, %d>" % ("Field10" if args.layout == lxv.LAYOUT_10X26 else "Field8", b.batch,
-                                             4 if wpc >= 16 else (3 if wpc >= 12 else (2 if wpc >= 8 else 1)))
+                                             4 if wpc >= 16 else (2 if wpc >= 8 else 1))
     if pmc and kname in pmc.get("kernel", "") and "hbm_bytes_per_key" in pmc:
         # bytes per launch of this run = measured bytes per key x keys per launch
         roofline["traffic"] = pmc["hbm_bytes_per_key"] * per_iter * iters

@@ -143,8 +143,6 @@ static const WalkEntry kWalks[] = {
     LXV_W3(Field8, LXV_LAYOUT_8X32, 32),    LXV_W3(Field8, LXV_LAYOUT_8X32, 64),
     LXV_W3(Field8, LXV_LAYOUT_8X32, 128),   LXV_W3(Field8, LXV_LAYOUT_8X32, 256),
 #undef LXV_W3
-    // 3 waves per SIMD (168 VGPRs): between the spill-free 2 and the spilling 4
-    LXV_WALK_ENTRY(Field8, LXV_LAYOUT_8X32, 64, 3), LXV_WALK_ENTRY(Field8, LXV_LAYOUT_8X32, 256, 3),
 };
 
 bool walk_supported(int layout, int batch, int waves) {

@@ -122,7 +122,7 @@ extern "C" int lxv_open(const lxv_open_opts* opts, lxv_ctx** out) {
     }
     c->batch = o.batch ? o.batch : 64;
     uint32_t wpc = o.waves_per_cu ? o.waves_per_cu : 16;
-    c->waves = wpc >= 16 ? 4 : (wpc >= 12 ? 3 : (wpc >= 8 ? 2 : 1));
+    c->waves = wpc >= 16 ? 4 : (wpc >= 8 ? 2 : 1);
     if (!walk_supported((int)c->layout, (int)c->batch, (int)c->waves)) {
         delete c;
         return fail(nullptr, LXV_E_ARG,
@@ -545,7 +545,7 @@ extern "C" int lxv_debug_copy(int device, uint64_t bytes, double* ms) {
 
 extern "C" int lxv_debug_kernel_attrs(int layout, uint32_t batch, uint32_t waves_per_cu, uint32_t* vgprs,
                                       uint32_t* sgprs, uint32_t* scratch_bytes, uint32_t* waves_per_simd) {
-    const int w = waves_per_cu >= 16 ? 4 : (waves_per_cu >= 12 ? 3 : (waves_per_cu >= 8 ? 2 : 1));
+    const int w = waves_per_cu >= 16 ? 4 : (waves_per_cu >= 8 ? 2 : 1);
     if (!walk_supported(layout, (int)batch, w)) return LXV_E_ARG;
     hipFuncAttributes a;
     if (walk_attributes(layout, (int)batch, w, &a) != hipSuccess) return LXV_E_HIP;
