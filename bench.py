@@ -41,10 +41,11 @@ import lxv  # noqa: E402
 METRIC = "keys/s on 1 MI355X vs plain HIP kernel; p50/p99 time-to-hit, mixed prefixes"
 
 
-def mad_work_per_key(batch):
+def mad_work_per_key(batch, affine=False):
     """32x32->64 products per key on the 8x32-bit schoolbook count:
-    M(B) = 8 + 11/B multiplications and S(B) = 2 + 254/B squarings per key."""
-    return 64 * (8 + 11 / batch) + 36 * (2 + 254 / batch)
+    M(B) = 8 + 11/B multiplications and S(B) = 2 + 254/B squarings per key
+    for the projective walk; the affine sub-walk form needs 6 + 11/B."""
+    return 64 * ((6 if affine else 8) + 11 / batch) + 36 * (2 + 254 / batch)
 
 
 def measure_mad_peak():
@@ -173,6 +174,8 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--layout", type=int, default=lxv.LAYOUT_8X32)
+    ap.add_argument("--subwalks", type=int, default=0,
+                    help="K affine sub-walks per lane, inverted across the K (0: one walk per lane)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one plain copy kernel of this many MiB (counter calibration)")
@@ -188,7 +191,7 @@ def main():
     start, ed_seed, prefix = lxv.seeded_search(1, 7)
     ed_pub = lxv.ed25519_public(ed_seed)
     search = lxv.GpuSearch(start=start, ed_seed=ed_seed, batch=args.batch, waves_per_cu=args.waves_per_cu,
-                           layout=args.layout)
+                           layout=args.layout, subwalks=args.subwalks)
     b = search.backend
     per_iter = b.threads * b.batch
     budget = 1 << args.budget_log2
@@ -218,24 +221,30 @@ def main():
     value = keys / gpu_s
 
     mad_peak, clock, cus = measure_mad_peak()
-    work = mad_work_per_key(b.batch)
+    kpi = args.subwalks or b.batch  # keys per inversion
+    work = mad_work_per_key(kpi, affine=bool(args.subwalks))
     achieved = value * work
     roofline = {"bound": "valu-imad", "achieved": achieved / 1e12, "peak": (mad_peak or 0) / 1e12,
                 "unit": "T mad/s", "frac": (achieved / mad_peak) if mad_peak else None,
                 "traffic": None, "mads_per_key": work}
     # HBM: the algorithmic bytes per key are the batch arrays (a_i and Z_i,
     # written in the forward pass and read back in the backward pass) plus
-    # the lane state read and written once per launch.
+    # the lane state read and written once per launch; in the sub-walk form,
+    # the affine state (u_n, u_{n-1}) read and u_{n+1} written every key.
     limbs = 10 if args.layout == lxv.LAYOUT_10X26 else 8
     fe_bytes = 4 * limbs
-    alg_bytes_per_key = 2 * 2 * fe_bytes + 2 * 4 * fe_bytes / (iters * b.batch)
+    if args.subwalks:
+        alg_bytes_per_key = 2 * 2 * fe_bytes + 3 * 32
+    else:
+        alg_bytes_per_key = 2 * 2 * fe_bytes + 2 * 4 * fe_bytes / (iters * b.batch)
     roofline["hbm"] = {"algorithmic_bytes_per_key": alg_bytes_per_key,
                        "achieved_GBps": value * alg_bytes_per_key / 1e9, "peak_GBps": 8000.0,
                        "frac": value * alg_bytes_per_key / 8e12}
     pmc = load_traffic()
     wpc = args.waves_per_cu or 16
-    kname = "walk_kernel<lxv::%s, %d, %d>" % ("Field10" if args.layout == lxv.LAYOUT_10X26 else "Field8", b.batch,
-                                             4 if wpc >= 16 else (2 if wpc >= 8 else 1))
+    kname = "%s<lxv::%s, %d, %d>" % ("walk_sw_kernel" if args.subwalks else "walk_kernel",
+                                    "Field10" if args.layout == lxv.LAYOUT_10X26 else "Field8", kpi,
+                                    4 if wpc >= 16 else (2 if wpc >= 8 else 1))
     if pmc and kname in pmc.get("kernel", "") and "hbm_bytes_per_key" in pmc:
         # bytes per launch of this run = measured bytes per key x keys per launch
         roofline["traffic"] = pmc["hbm_bytes_per_key"] * per_iter * iters
@@ -255,9 +264,11 @@ def main():
         "warmup": args.warmup, "ms_per_step": sum(kernel_ms) / len(kernel_ms), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u32 (exact integer limbs)",
         "data": "synthetic (seeded start scalar, Ed25519 key and prefix)",
-        "config": {"workload": "plain HIP walk, single search, 7-hex prefix, B=%d, 2^%d-key budget"
-                               % (b.batch, args.budget_log2),
-                   "lanes": b.threads, "keys_per_inversion": b.batch, "iters_per_step": iters,
+        "config": {"workload": ("plain HIP walk, single search, 7-hex prefix, B=%d, 2^%d-key budget"
+                                % (b.batch, args.budget_log2)) if not args.subwalks else
+                               ("sub-walk form: %d affine walks per lane inverted together, single search, 7-hex "
+                                "prefix, 2^%d-key budget" % (args.subwalks, args.budget_log2)),
+                   "threads": b.threads, "keys_per_inversion": kpi, "iters_per_step": iters,
                    "prefix": prefix, "layout": "10x25.5" if args.layout == lxv.LAYOUT_10X26 else "8x32",
                    "device": b.describe().strip()},
         "keys": keys, "wall_s": wall, "hits": len(hits), "hits_rechecked": rechecked,

@@ -57,6 +57,13 @@ typedef struct lxv_open_opts {
     uint32_t batch;        /* keys per batched inversion (B); 0 = 64 */
     uint32_t waves_per_cu; /* resident waves per CU the grid is sized for; 0 = 16 */
     uint32_t layout;       /* LXV_LAYOUT_*; 0 (8x32) is the default */
+    uint32_t subwalks;     /* 0: one walk per lane, B keys per inversion (the reference's
+                              form). K = 64, 128 or 256 (8x32 only): K walks per lane in
+                              affine coordinates, inverted across the K, which saves two
+                              of the eight multiplications per key; then a thread is one
+                              of those walks (lxv_threads = lanes x K), lxv_batch is 1 (one
+                              key per thread per iteration) and `threads` above counts
+                              lanes */
 } lxv_open_opts;
 
 /* What one launch needs besides the per-lane state: LaunchArgs,

@@ -42,6 +42,17 @@ hipError_t launch_init_ladder(int layout, uint32_t* state, uint64_t* ns, uint32_
                               const uint32_t q[8], uint64_t stride, uint32_t lanes, hipStream_t s);
 hipError_t launch_clear(uint32_t* words, uint32_t n, hipStream_t s);
 
+// The sub-walk form (lxv_subwalk.h): `lanes` physical lanes of `subwalks`
+// affine walks each; the state holds lanes x subwalks x 64 bytes.
+bool walk_sw_supported(int layout, int subwalks, int waves);
+hipError_t launch_walk_sw(int layout, int subwalks, int waves, const WalkParams& p, hipStream_t s);
+hipError_t walk_sw_attributes(int layout, int subwalks, int waves, hipFuncAttributes* out);
+hipError_t launch_init_sw_from_u(int layout, uint32_t* state, uint64_t* ns, uint32_t* xor_acc, const uint32_t* upc,
+                                 uint32_t lanes, uint32_t subwalks, hipStream_t s);
+hipError_t launch_init_sw_ladder(int layout, uint32_t* state, uint64_t* ns, uint32_t* xor_acc, const uint32_t base[8],
+                                 const uint32_t q[8], uint64_t stride, uint32_t lanes, uint32_t subwalks,
+                                 hipStream_t s);
+
 struct MultiParams;
 struct MultiInitEntry;
 bool multi_supported(int layout, int batch);

@@ -8,6 +8,8 @@
 //   init_ladder    lane state from an on-device X25519 ladder
 //   clear          zero the per-launch counters (no hipMemsetAsync, so the
 //                  launch sequence can be captured in a graph)
+//   walk_sw        the sub-walk form: K affine walks per lane, inverted across
+//                  (lxv_subwalk.h), with its two initialisers
 //   debug kernels  field operations and the address hash on given inputs, for
 //                  parity tests against the CPU
 #include "fe10.h"
@@ -15,6 +17,7 @@
 #include "lxv_kernels.h"
 #include "lxv_walk.h"
 #include "lxv_multi.h"
+#include "lxv_subwalk.h"
 
 namespace lxv {
 
@@ -43,6 +46,26 @@ __global__ __launch_bounds__(64) void init_ladder_kernel(uint32_t* state, uint64
                                                          const LadderArgs a) {
     const uint32_t lane = blockIdx.x * 64u + threadIdx.x;
     if (lane < a.lanes) init_lane_ladder<F>(state, ns, xor_acc, a.base, a.stride, a.q, a.lanes, lane);
+}
+
+template <class F, int K, int W>
+__global__ __launch_bounds__(64, W) void walk_sw_kernel(const WalkParams p) {
+    const uint32_t lane = blockIdx.x * 64u + threadIdx.x;
+    if (lane < p.lanes) walk_sw_lane<F, K>(p, lane);
+}
+
+template <class F>
+__global__ __launch_bounds__(64) void init_sw_from_u_kernel(uint2* st, uint64_t* ns, uint32_t* xor_acc,
+                                                            const uint32_t* upc, uint32_t lanes, uint32_t K) {
+    const uint32_t t = blockIdx.x * 64u + threadIdx.x;
+    if (t < lanes * K) init_sw_from_u<F>(st, ns, xor_acc, upc, lanes, K, t);
+}
+
+template <class F>
+__global__ __launch_bounds__(64) void init_sw_ladder_kernel(uint2* st, uint64_t* ns, uint32_t* xor_acc,
+                                                            const LadderArgs a, uint32_t K) {
+    const uint32_t t = blockIdx.x * 64u + threadIdx.x;
+    if (t < a.lanes * K) init_sw_ladder<F>(st, ns, xor_acc, a.base, a.stride, a.q, a.lanes, K, t);
 }
 
 __global__ void clear_kernel(uint32_t* words, uint32_t n) {
@@ -195,6 +218,68 @@ hipError_t launch_init_ladder(int layout, uint32_t* state, uint64_t* ns, uint32_
         hipLaunchKernelGGL(init_ladder_kernel<Field8>, grid, block, 0, s, state, ns, xor_acc, a);
     else
         hipLaunchKernelGGL(init_ladder_kernel<Field10>, grid, block, 0, s, state, ns, xor_acc, a);
+    return hipGetLastError();
+}
+
+// The sub-walk form: 8 x 32-bit limbs, K sub-walks per lane.
+struct SwEntry {
+    int layout;
+    int subwalks;
+    int waves;
+    walk_fn fn;
+};
+static const SwEntry kSwWalks[] = {
+    {LXV_LAYOUT_8X32, 64, 2, walk_sw_kernel<Field8, 64, 2>},   {LXV_LAYOUT_8X32, 64, 4, walk_sw_kernel<Field8, 64, 4>},
+    {LXV_LAYOUT_8X32, 128, 2, walk_sw_kernel<Field8, 128, 2>}, {LXV_LAYOUT_8X32, 128, 4, walk_sw_kernel<Field8, 128, 4>},
+    {LXV_LAYOUT_8X32, 256, 2, walk_sw_kernel<Field8, 256, 2>}, {LXV_LAYOUT_8X32, 256, 4, walk_sw_kernel<Field8, 256, 4>},
+};
+
+bool walk_sw_supported(int layout, int subwalks, int waves) {
+    for (const auto& e : kSwWalks)
+        if (e.layout == layout && e.subwalks == subwalks && e.waves == waves) return true;
+    return false;
+}
+
+hipError_t launch_walk_sw(int layout, int subwalks, int waves, const WalkParams& p, hipStream_t s) {
+    for (const auto& e : kSwWalks) {
+        if (e.layout == layout && e.subwalks == subwalks && e.waves == waves) {
+            hipLaunchKernelGGL(e.fn, dim3((p.lanes + 63) / 64), dim3(64), 0, s, p);
+            return hipGetLastError();
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t walk_sw_attributes(int layout, int subwalks, int waves, hipFuncAttributes* out) {
+    for (const auto& e : kSwWalks)
+        if (e.layout == layout && e.subwalks == subwalks && e.waves == waves)
+            return hipFuncGetAttributes(out, reinterpret_cast<const void*>(e.fn));
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_init_sw_from_u(int layout, uint32_t* state, uint64_t* ns, uint32_t* xor_acc, const uint32_t* upc,
+                                 uint32_t lanes, uint32_t subwalks, hipStream_t s) {
+    if (layout != LXV_LAYOUT_8X32) return hipErrorInvalidValue;
+    const uint64_t total = (uint64_t)lanes * subwalks;
+    hipLaunchKernelGGL(init_sw_from_u_kernel<Field8>, dim3((uint32_t)((total + 63) / 64)), dim3(64), 0, s,
+                       reinterpret_cast<uint2*>(state), ns, xor_acc, upc, lanes, subwalks);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_sw_ladder(int layout, uint32_t* state, uint64_t* ns, uint32_t* xor_acc, const uint32_t base[8],
+                                 const uint32_t q[8], uint64_t stride, uint32_t lanes, uint32_t subwalks,
+                                 hipStream_t s) {
+    if (layout != LXV_LAYOUT_8X32) return hipErrorInvalidValue;
+    LadderArgs a;
+    for (int i = 0; i < 8; i++) {
+        a.base[i] = base[i];
+        a.q[i] = q[i];
+    }
+    a.stride = stride;
+    a.lanes = lanes;
+    const uint64_t total = (uint64_t)lanes * subwalks;
+    hipLaunchKernelGGL(init_sw_ladder_kernel<Field8>, dim3((uint32_t)((total + 63) / 64)), dim3(64), 0, s,
+                       reinterpret_cast<uint2*>(state), ns, xor_acc, a, subwalks);
     return hipGetLastError();
 }
 

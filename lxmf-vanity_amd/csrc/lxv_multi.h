@@ -177,40 +177,8 @@ LXV_D void multi_wave(const MultiParams& p, uint32_t plane) {
 template <class F>
 LXV_D void multi_init_lane(uint32_t* pool, uint32_t pool_lanes, uint32_t state_off, const uint32_t base[8],
                            uint64_t stride, const uint32_t qw[8], uint32_t v) {
-    typedef typename F::fe fe;
-    uint32_t m[8];
-    const uint64_t off = (uint64_t)v * stride;
-    uint64_t carry = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        const uint64_t add = i == 0 ? (off & 0xffffffffu) : (i == 1 ? (off >> 32) : 0);
-        const uint64_t s = (uint64_t)base[i] + add + carry;
-        m[i] = (uint32_t)s;
-        carry = s >> 32;
-    }
-    const fe x1 = F::from_words(qw);
-    fe x2 = F::one(), z2 = F::zero(), x3 = x1, z3 = F::one();
-    uint32_t swap = 0;
-    for (int wi = 7; wi >= 0; wi--) {
-        const uint32_t word = m[wi];
-        for (int bi = 31; bi >= 0; bi--) {
-            const uint32_t bit = (word >> bi) & 1u;
-            swap ^= bit;
-            F::cswap(x2, x3, swap);
-            F::cswap(z2, z3, swap);
-            swap = bit;
-            const fe a = F::add(x2, z2), aa = F::sq(a), b = F::sub(x2, z2), bb = F::sq(b);
-            const fe e = F::sub(aa, bb);
-            const fe c = F::add(x3, z3), d = F::sub(x3, z3);
-            const fe da = F::mul(d, a), cb = F::mul(c, b);
-            x3 = F::sq(F::add(da, cb));
-            z3 = F::mul(x1, F::sq(F::sub(da, cb)));
-            x2 = F::mul(aa, bb);
-            z2 = F::mul(e, F::add(aa, F::mul_small(e, 121665)));
-        }
-    }
-    F::cswap(x2, x3, swap);
-    F::cswap(z2, z3, swap);
+    typename F::fe x2, z2, x3, z3;
+    ladder_mq<F>(base, (uint64_t)v * stride, qw, x2, z2, x3, z3);
     const uint32_t lane = state_off + v;
     store_state<F>(pool, pool_lanes, lane, 0, x2);
     store_state<F>(pool, pool_lanes, lane, 1, z2);

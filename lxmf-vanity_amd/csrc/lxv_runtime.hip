@@ -36,6 +36,8 @@ struct lxv_ctx {
     std::string desc;
     std::string err = "no error";
     uint32_t lanes = 0, batch = 64, waves = 4, layout = LXV_LAYOUT_10X26, limbs = 10;
+    uint32_t subwalks = 0;  // 0: one projective walk per lane; K: K affine sub-walks per lane
+    uint32_t phys = 0;      // physical lanes (= lanes unless sub-walks)
     uint32_t* d_state = nullptr;
     uint64_t* d_ns = nullptr;
     uint32_t* d_xor = nullptr;
@@ -120,17 +122,34 @@ extern "C" int lxv_open(const lxv_open_opts* opts, lxv_ctx** out) {
         delete c;
         return fail(nullptr, LXV_E_ARG, "lxv_open: unknown layout");
     }
-    c->batch = o.batch ? o.batch : 64;
     uint32_t wpc = o.waves_per_cu ? o.waves_per_cu : 16;
     c->waves = wpc >= 16 ? 4 : (wpc >= 8 ? 2 : 1);
-    if (!walk_supported((int)c->layout, (int)c->batch, (int)c->waves)) {
+    c->subwalks = o.subwalks;
+    if (c->subwalks) {
+        // K sub-walks per lane: a thread is a sub-walk, one key per iteration
+        if (o.batch > 1 || !walk_sw_supported((int)c->layout, (int)c->subwalks, (int)c->waves)) {
+            delete c;
+            return fail(nullptr, LXV_E_ARG,
+                        "lxv_open: sub-walks need the 8x32 layout, subwalks 64, 128 or 256, batch 0 or 1");
+        }
+        c->batch = 1;
+    } else {
+        c->batch = o.batch ? o.batch : 64;
+    }
+    if (!c->subwalks && !walk_supported((int)c->layout, (int)c->batch, (int)c->waves)) {
         delete c;
         return fail(nullptr, LXV_E_ARG,
                     "lxv_open: no walk kernel built for this layout / batch / waves (batch must be 16..256, a power of two)");
     }
     c->limbs = (uint32_t)limbs_of((int)c->layout);
-    c->lanes = o.threads ? o.threads : (uint32_t)prop.multiProcessorCount * wpc * 64u;
-    c->lanes = (c->lanes + 63) / 64 * 64;
+    c->phys = o.threads ? o.threads : (uint32_t)prop.multiProcessorCount * wpc * 64u;
+    c->phys = (c->phys + 63) / 64 * 64;
+    const uint64_t threads = (uint64_t)c->phys * (c->subwalks ? c->subwalks : 1);
+    if (threads > 0xffffffffull) {
+        delete c;
+        return fail(nullptr, LXV_E_ARG, "lxv_open: lanes x subwalks exceeds 2^32");
+    }
+    c->lanes = (uint32_t)threads;
     char buf[256];
     snprintf(buf, sizeof buf, "%s%s%s, %d CUs (HIP)", prop.name, prop.name[0] ? " " : "AMD GPU ", prop.gcnArchName, prop.multiProcessorCount);
     c->desc = buf;
@@ -144,11 +163,17 @@ extern "C" int lxv_open(const lxv_open_opts* opts, lxv_ctx** out) {
             return fail(nullptr, LXV_E_HIP, m);                         \
         }                                                               \
     } while (0)
-    const size_t L = c->lanes;
-    ALLOC(c->d_state, L * 4 * c->limbs * sizeof(uint32_t));
+    const size_t L = c->phys;
+    if (c->subwalks) {
+        // two slots of 8 canonical words per sub-walk; a_j and D_j per sub-walk
+        ALLOC(c->d_state, L * c->subwalks * 2 * 8 * sizeof(uint32_t));
+        ALLOC(c->d_batch, L * (size_t)c->subwalks * 2 * c->limbs * sizeof(uint32_t));
+    } else {
+        ALLOC(c->d_state, L * 4 * c->limbs * sizeof(uint32_t));
+        ALLOC(c->d_batch, L * (size_t)c->batch * 2 * c->limbs * sizeof(uint32_t));
+    }
     ALLOC(c->d_ns, L * sizeof(uint64_t));
     ALLOC(c->d_xor, L * 4 * sizeof(uint32_t));
-    ALLOC(c->d_batch, L * (size_t)c->batch * 2 * c->limbs * sizeof(uint32_t));
     ALLOC(c->d_counters, 16 * sizeof(uint32_t));
 #undef ALLOC
     if ((e = hipHostMalloc((void**)&c->h_stop, 64, hipHostMallocMapped | hipHostMallocCoherent)) != hipSuccess) {
@@ -186,7 +211,11 @@ extern "C" int lxv_init(lxv_ctx* c, const uint32_t* upc, size_t n_words) {
     uint32_t* d_upc = nullptr;
     HIPCHK(c, hipMalloc((void**)&d_upc, n_words * sizeof(uint32_t)));
     hipError_t e = hipMemcpyAsync(d_upc, upc, n_words * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = launch_init_from_u((int)c->layout, c->d_state, c->d_ns, c->d_xor, d_upc, c->lanes, c->stream);
+    if (e == hipSuccess)
+        e = c->subwalks ? launch_init_sw_from_u((int)c->layout, c->d_state, c->d_ns, c->d_xor, d_upc, c->phys,
+                                                c->subwalks, c->stream)
+                        : launch_init_from_u((int)c->layout, c->d_state, c->d_ns, c->d_xor, d_upc, c->lanes,
+                                             c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d_upc);
     if (e != hipSuccess) return hip_fail(c, e, "lxv_init");
@@ -220,30 +249,37 @@ extern "C" int lxv_init_seeded(lxv_ctx* c, const uint8_t start[32], uint64_t str
     uint8_t zero[32] = {0};
     lxv_launch_args_for(zero, &a);
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, launch_init_ladder((int)c->layout, c->d_state, c->d_ns, c->d_xor, base, a.q, stride, c->lanes, c->stream));
+    if (c->subwalks)
+        HIPCHK(c, launch_init_sw_ladder((int)c->layout, c->d_state, c->d_ns, c->d_xor, base, a.q, stride, c->phys,
+                                        c->subwalks, c->stream));
+    else
+        HIPCHK(c, launch_init_ladder((int)c->layout, c->d_state, c->d_ns, c->d_xor, base, a.q, stride, c->lanes,
+                                     c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->initialized = true;
     return LXV_OK;
 }
 
 template <class F>
-static void limbs_of_q(const uint32_t q[8], uint32_t qp[10], uint32_t qm[10]) {
+static void limbs_of_q(const uint32_t q[8], uint32_t qp[10], uint32_t qm[10], uint32_t qr[10]) {
     typedef typename F::fe fe;
     const fe qf = F::from_words(q);
-    uint32_t a[F::kLimbs], b[F::kLimbs];
+    uint32_t a[F::kLimbs], b[F::kLimbs], r[F::kLimbs];
     F::store(F::add(qf, F::one()), a);
     F::store(F::sub(qf, F::one()), b);
+    F::store(qf, r);
     for (int i = 0; i < 10; i++) {
         qp[i] = i < F::kLimbs ? a[i] : 0;
         qm[i] = i < F::kLimbs ? b[i] : 0;
+        qr[i] = i < F::kLimbs ? r[i] : 0;
     }
 }
 
-static void fill_q(uint32_t layout, const uint32_t q[8], uint32_t qp[10], uint32_t qm[10]) {
+static void fill_q(uint32_t layout, const uint32_t q[8], uint32_t qp[10], uint32_t qm[10], uint32_t qr[10]) {
     if (layout == LXV_LAYOUT_8X32)
-        limbs_of_q<Field8>(q, qp, qm);
+        limbs_of_q<Field8>(q, qp, qm, qr);
     else
-        limbs_of_q<Field10>(q, qp, qm);
+        limbs_of_q<Field10>(q, qp, qm, qr);
 }
 
 extern "C" int lxv_launch(lxv_ctx* c, const lxv_launch_args* a) {
@@ -271,7 +307,7 @@ extern "C" int lxv_launch(lxv_ctx* c, const lxv_launch_args* a) {
     WalkParams p;
     std::memset(&p, 0, sizeof p);
     sha_search_init(p.sha, a->ed, a->name_hash, a->want, a->mask);
-    fill_q(c->layout, a->q, p.qp, p.qm);
+    fill_q(c->layout, a->q, p.qp, p.qm, p.q);
     p.state = c->d_state;
     p.ns = c->d_ns;
     p.xor_acc = c->d_xor;
@@ -284,12 +320,15 @@ extern "C" int lxv_launch(lxv_ctx* c, const lxv_launch_args* a) {
     p.stop = c->d_stop;
     p.max_hits = a->max_hits;
     p.dump_threads = a->dump_threads;
-    p.lanes = c->lanes;
+    p.lanes = c->phys;
     p.iters = a->iters;
     *(volatile uint32_t*)c->h_stop = 0;
     HIPCHK(c, launch_clear(c->d_counters, 16, c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    HIPCHK(c, launch_walk((int)c->layout, (int)c->batch, (int)c->waves, p, c->stream));
+    if (c->subwalks)
+        HIPCHK(c, launch_walk_sw((int)c->layout, (int)c->subwalks, (int)c->waves, p, c->stream));
+    else
+        HIPCHK(c, launch_walk((int)c->layout, (int)c->batch, (int)c->waves, p, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->last = *a;
     c->in_flight = true;
@@ -343,12 +382,12 @@ extern "C" int lxv_checksum(lxv_ctx* c, uint32_t xw[4]) {
     int rc = check_idle(c, "lxv_checksum");
     if (rc) return rc;
     if (!xw) return fail(c, LXV_E_ARG, "lxv_checksum: NULL output");
-    std::vector<uint32_t> h((size_t)4 * c->lanes);
+    std::vector<uint32_t> h((size_t)4 * c->phys);
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipMemcpy(h.data(), c->d_xor, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
     for (int k = 0; k < 4; k++) {
         uint32_t x = 0;
-        for (size_t i = 0; i < c->lanes; i++) x ^= h[(size_t)k * c->lanes + i];
+        for (size_t i = 0; i < c->phys; i++) x ^= h[(size_t)k * c->phys + i];
         xw[k] = x;
     }
     return LXV_OK;
@@ -359,7 +398,14 @@ extern "C" int lxv_read_ns(lxv_ctx* c, uint32_t first, uint32_t count, uint64_t*
     if (rc) return rc;
     if (!ns || (uint64_t)first + count > c->lanes) return fail(c, LXV_E_ARG, "lxv_read_ns: range out of bounds");
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipMemcpy(ns, c->d_ns + first, (size_t)count * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (!c->subwalks) {
+        HIPCHK(c, hipMemcpy(ns, c->d_ns + first, (size_t)count * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        return LXV_OK;
+    }
+    // every sub-walk of a lane is at the lane's n: thread t is lane t % lanes
+    std::vector<uint64_t> h(c->phys);
+    HIPCHK(c, hipMemcpy(h.data(), c->d_ns, h.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < count; i++) ns[i] = h[(first + i) % c->phys];
     return LXV_OK;
 }
 

@@ -24,6 +24,7 @@ struct WalkParams {
     ShaSearch sha;
     uint32_t qp[10];             // u(8B) + 1 as raw limbs of the layout
     uint32_t qm[10];             // u(8B) - 1 as raw limbs of the layout
+    uint32_t q[10];              // u(8B) as raw limbs of the layout (sub-walk form)
     uint32_t* state;             // [4 * kLimbs][lanes]: X(n-1), Z(n-1), X(n), Z(n)
     uint64_t* ns;                // [lanes]: n of the lane's next key
     uint32_t* xor_acc;           // [4][lanes]: XOR of every address walked
@@ -218,18 +219,13 @@ LXV_D void init_lane_from_u(uint32_t* state, uint64_t* ns, uint32_t* xor_acc, co
     for (int k = 0; k < 4; k++) xor_acc[(size_t)k * lanes + lane] = 0;
 }
 
-// State of lane t by a Montgomery ladder on the device: with k = start + 8 t
-// stride and m = k / 8 - 1, the ladder over Q = 8B gives (X:Z) of mQ =
-// u(k - 8) and (m + 1)Q = u(k) with difference Q, which is exactly the
-// walk's (P(n-1), P(n)) in projective form, so no inversion is needed.
-// base = start / 8 - 1 as 8 little-endian words (start is a multiple of 8
-// with bit 254 set, so this never wraps); lane * stride < 2^63.
+// The X25519 ladder over Q = u(qw) for the scalar m = base + off (base as 8
+// little-endian words, off < 2^63, no wrap): (X:Z) of mQ and (m + 1)Q.
 template <class F>
-LXV_D void init_lane_ladder(uint32_t* state, uint64_t* ns, uint32_t* xor_acc, const uint32_t base[8],
-                            uint64_t stride, const uint32_t qw[8], uint32_t lanes, uint32_t lane) {
+LXV_D void ladder_mq(const uint32_t base[8], uint64_t off, const uint32_t qw[8], typename F::fe& x2,
+                     typename F::fe& z2, typename F::fe& x3, typename F::fe& z3) {
     typedef typename F::fe fe;
     uint32_t m[8];
-    const uint64_t off = (uint64_t)lane * stride;
     uint64_t carry = 0;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
@@ -239,7 +235,10 @@ LXV_D void init_lane_ladder(uint32_t* state, uint64_t* ns, uint32_t* xor_acc, co
         carry = v >> 32;
     }
     const fe x1 = F::from_words(qw);
-    fe x2 = F::one(), z2 = F::zero(), x3 = x1, z3 = F::one();
+    x2 = F::one();
+    z2 = F::zero();
+    x3 = x1;
+    z3 = F::one();
     uint32_t swap = 0;
     for (int wi = 7; wi >= 0; wi--) {
         const uint32_t word = m[wi];
@@ -261,6 +260,19 @@ LXV_D void init_lane_ladder(uint32_t* state, uint64_t* ns, uint32_t* xor_acc, co
     }
     F::cswap(x2, x3, swap);
     F::cswap(z2, z3, swap);
+}
+
+// State of lane t by a Montgomery ladder on the device: with k = start + 8 t
+// stride and m = k / 8 - 1, the ladder over Q = 8B gives (X:Z) of mQ =
+// u(k - 8) and (m + 1)Q = u(k) with difference Q, which is exactly the
+// walk's (P(n-1), P(n)) in projective form, so no inversion is needed.
+// base = start / 8 - 1 as 8 little-endian words (start is a multiple of 8
+// with bit 254 set, so this never wraps); lane * stride < 2^63.
+template <class F>
+LXV_D void init_lane_ladder(uint32_t* state, uint64_t* ns, uint32_t* xor_acc, const uint32_t base[8],
+                            uint64_t stride, const uint32_t qw[8], uint32_t lanes, uint32_t lane) {
+    typename F::fe x2, z2, x3, z3;
+    ladder_mq<F>(base, (uint64_t)lane * stride, qw, x2, z2, x3, z3);
     store_state<F>(state, lanes, lane, 0, x2);
     store_state<F>(state, lanes, lane, 1, z2);
     store_state<F>(state, lanes, lane, 2, x3);

@@ -39,7 +39,8 @@ class GpuUnavailable(RuntimeError):
 
 class OpenOpts(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("device", ctypes.c_int32), ("threads", ctypes.c_uint32),
-                ("batch", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32), ("layout", ctypes.c_uint32)]
+                ("batch", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32), ("layout", ctypes.c_uint32),
+                ("subwalks", ctypes.c_uint32)]
 
 
 class LaunchArgs(ctypes.Structure):
@@ -218,8 +219,12 @@ def words_to_bytes_le(words):
 class Backend:
     """One GPU context: Backend::{describe, threads, batch, init, launch}."""
 
-    def __init__(self, threads=0, batch=64, waves_per_cu=0, layout=LAYOUT_8X32, device=0):
-        o = OpenOpts(ctypes.sizeof(OpenOpts), device, threads, batch, waves_per_cu, layout)
+    def __init__(self, threads=0, batch=64, waves_per_cu=0, layout=LAYOUT_8X32, device=0, subwalks=0):
+        """subwalks=K: K affine sub-walks per lane (threads then counts lanes
+        and self.threads is lanes x K; batch is 1 key per thread per launch
+        iteration)."""
+        o = OpenOpts(ctypes.sizeof(OpenOpts), device, threads, 0 if subwalks else batch, waves_per_cu, layout,
+                     subwalks)
         h = ctypes.c_void_p()
         rc = lib().lxv_open(ctypes.byref(o), ctypes.byref(h))
         if rc != LXV_OK:
@@ -304,6 +309,9 @@ class GpuSearch:
         if not start_scalar_ok(self.start):
             raise ValueError("not a start scalar random_start_scalar could return")
         self.seed = ed_seed if ed_seed is not None else os.urandom(32)
+        # threads x stride must stay below 2^63 (many sub-walks: a smaller stride)
+        while stride > 1 and self.backend.threads * stride >= 1 << 63:
+            stride >>= 1
         self.stride = stride
         self.backend.init_seeded(self.start, stride)
         self.ed_pub = ed25519_public(self.seed)

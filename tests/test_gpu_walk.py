@@ -68,9 +68,18 @@ def test_gpu_address_hash_matches_oracle():
         assert lxv.words_to_bytes_be(out[4 * i:4 * i + 4]) == orc.lxmf_address(u, ed_pub)
 
 
-@pytest.fixture(scope="module", params=[lxv.LAYOUT_10X26, lxv.LAYOUT_8X32], ids=["10x26", "8x32"])
+# The reference's form (one projective walk per lane, 64 keys per inversion)
+# in both limb layouts, and the sub-walk form (64 affine walks per lane: a
+# thread is one of them, one key per thread per iteration).
+FORMS = {"10x26": dict(threads=4096, batch=64, layout=lxv.LAYOUT_10X26),
+         "8x32": dict(threads=4096, batch=64, layout=lxv.LAYOUT_8X32),
+         "8x32-sw64": dict(threads=512, subwalks=64, layout=lxv.LAYOUT_8X32)}
+
+
+@pytest.fixture(scope="module", params=sorted(FORMS))
 def small_backend(request):
-    b = lxv.Backend(threads=4096, batch=64, layout=request.param)
+    print("backend", request.param, flush=True)
+    b = lxv.Backend(**FORMS[request.param])
     yield b
     b.close()
 
@@ -80,12 +89,14 @@ def test_walk_dump_matches_x25519(small_backend):
     start, _, ed_pub, _ = seeded(41, 2)
     b.init_seeded(start, STRIDE)
     hits, dump = b.launch(lxv.launch_args_for(ed_pub, "00", iters=1, dump_threads=256))
+    B = b.batch
+    assert len(dump) == 256 * B * 8
     rng = random.Random(42)
     lanes = [0, 1, 63, 64, 255] + rng.sample(range(256), 20)
     for t in lanes:
-        for i in list(range(8)) + [31, 32, 62, 63]:
+        for i in [i for i in list(range(8)) + [31, 32, 62, 63] if i < B]:
             k = orc.step_scalar(start, t * STRIDE + i)
-            got = lxv.words_to_bytes_le(dump[(t * 64 + i) * 8:(t * 64 + i) * 8 + 8])
+            got = lxv.words_to_bytes_le(dump[(t * B + i) * 8:(t * B + i) * 8 + 8])
             assert got == orc.x25519(k), (t, i)
 
 
@@ -124,10 +135,11 @@ def test_walk_continues_across_launches(small_backend):
     assert b.checksum() == x_two
 
 
-def test_reference_style_init_from_host_start_points():
-    # Backend::init with u(k-8), u(k) per lane from independent random starts
-    # (src/gpu/mod.rs:210-228), computed with the product's host X25519.
-    b = lxv.Backend(threads=256, batch=64)
+@pytest.mark.parametrize("form", [dict(threads=256, batch=64), dict(threads=64, subwalks=64)], ids=["lanes", "sw64"])
+def test_reference_style_init_from_host_start_points(form):
+    # Backend::init with u(k-8), u(k) per thread from independent random
+    # starts (src/gpu/mod.rs:210-228), computed with the product's host X25519.
+    b = lxv.Backend(**form)
     try:
         rng = random.Random(48)
         starts = [tb.clamp(bytes(rng.getrandbits(8) for _ in range(32))) for _ in range(b.threads)]
@@ -141,10 +153,12 @@ def test_reference_style_init_from_host_start_points():
                 upc += [int.from_bytes(u[4 * i:4 * i + 4], "little") for i in range(8)]
         b.init(upc)
         ed_pub = lxv.ed25519_public(bytes(32))
-        hits, dump = b.launch(lxv.launch_args_for(ed_pub, "0", iters=1, dump_threads=b.threads, max_hits=1 << 14))
+        B = b.batch
+        hits, dump = b.launch(lxv.launch_args_for(ed_pub, "0", iters=max(1, 64 // B), dump_threads=b.threads,
+                                                  max_hits=1 << 14))
         for t in range(0, b.threads, 7):
-            for i in (0, 1, 33, 63):
-                got = lxv.words_to_bytes_le(dump[(t * 64 + i) * 8:(t * 64 + i) * 8 + 8])
+            for i in [i for i in (0, 1, 33, 63) if i < B]:
+                got = lxv.words_to_bytes_le(dump[(t * B + i) * 8:(t * B + i) * 8 + 8])
                 assert got == orc.x25519(orc.step_scalar(starts[t], i)), (t, i)
         for t, n, addr in hits:
             assert addr == orc.lxmf_address(orc.x25519(orc.step_scalar(starts[t], n)), ed_pub)
@@ -160,7 +174,7 @@ def test_full_address_prefix_finds_exactly_that_key(small_backend):
     t, n = 1234, 37
     want = orc.lxmf_address(orc.x25519(orc.step_scalar(start, t * STRIDE + n)), ed_pub)
     b.init_seeded(start, STRIDE)
-    hits, _ = b.launch(lxv.launch_args_for(ed_pub, want.hex(), iters=1))
+    hits, _ = b.launch(lxv.launch_args_for(ed_pub, want.hex(), iters=n // b.batch + 1))
     assert hits == [(t, n, want)]
 
 
@@ -168,7 +182,8 @@ def test_hits_beyond_max_hits_are_counted_not_stored(small_backend):
     b = small_backend
     start, _, ed_pub, _ = seeded(50, 0)
     b.init_seeded(start, STRIDE)
-    hits, _ = b.launch(lxv.launch_args_for(ed_pub, "0", iters=1, max_hits=100))
+    iters = max(1, (1 << 18) // (b.threads * b.batch))
+    hits, _ = b.launch(lxv.launch_args_for(ed_pub, "0", iters=iters, max_hits=100))
     st = b.stats()
     assert len(hits) == 100 and st.hits_total > 10000
 
@@ -182,34 +197,39 @@ def test_stop_ends_a_long_launch_and_the_walk_resumes(small_backend):
     b.stop()
     hits, _ = b.poll(True)
     st = b.stats()
-    assert st.stopped == 1 and 0 < st.keys < b.threads * 64 * 20000
+    B = b.batch
+    assert st.stopped == 1 and 0 < st.keys < b.threads * B * 20000
     ns = b.read_ns()
-    assert all(n % 64 == 0 for n in ns) and sum(ns) == st.keys
+    assert all(n % B == 0 for n in ns) and sum(ns) == st.keys
     # The walk resumes where each lane stopped: the next batch's keys match.
     h2, dump = b.launch(lxv.launch_args_for(ed_pub, prefix, iters=1, dump_threads=16))
     for t in range(16):
-        for i in (0, 63):
+        for i in sorted({0, B - 1}):
             k = orc.step_scalar(start, t * STRIDE + ns[t] + i)
-            assert lxv.words_to_bytes_le(dump[(t * 64 + i) * 8:(t * 64 + i) * 8 + 8]) == orc.x25519(k)
+            assert lxv.words_to_bytes_le(dump[(t * B + i) * 8:(t * B + i) * 8 + 8]) == orc.x25519(k)
     for t, n, addr in hits + h2:
         assert addr == orc.lxmf_address(orc.x25519(orc.step_scalar(start, t * STRIDE + n)), ed_pub)
 
 
-@pytest.mark.parametrize("layout", [lxv.LAYOUT_8X32, lxv.LAYOUT_10X26], ids=["8x32", "10x26"])
-def test_config0_workload_matches_golden(layout):
+@pytest.mark.parametrize("form", [dict(threads=65536, batch=256, layout=lxv.LAYOUT_8X32),
+                                  dict(threads=65536, batch=256, layout=lxv.LAYOUT_10X26),
+                                  dict(threads=1024, subwalks=64)], ids=["8x32", "10x26", "8x32-sw64"])
+def test_config0_workload_matches_golden(form):
     # The small workload at full size: 2^24 consecutive keys of workload
-    # seed 0 as 65536 lanes x 256 keys (lane t walks g in [256 t, 256 t + 256)),
-    # every hit of the 4-hex prefix and the XOR of all 2^24 addresses against
-    # the committed fixture.
+    # seed 0 split evenly over the threads (thread t walks g in [m t, m t + m)
+    # with m = 2^24 / threads), every hit of the 4-hex prefix and the XOR of
+    # all 2^24 addresses against the committed fixture.
     with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config0.json")) as f:
         fx = json.load(f)
     start, ed_pub = bytes.fromhex(fx["start"]), bytes.fromhex(fx["ed_pub"])
-    b = lxv.Backend(threads=65536, batch=256, layout=layout)
+    b = lxv.Backend(**form)
     try:
-        b.init_seeded(start, 256)
-        hits, _ = b.launch(lxv.launch_args_for(ed_pub, fx["prefix"], iters=1, max_hits=1 << 12))
+        per_thread = fx["keys"] // b.threads
+        assert per_thread * b.threads == fx["keys"]
+        b.init_seeded(start, per_thread)
+        hits, _ = b.launch(lxv.launch_args_for(ed_pub, fx["prefix"], iters=per_thread // b.batch, max_hits=1 << 12))
         assert b.stats().keys == fx["keys"]
-        got = sorted((t * 256 + n, a.hex()) for t, n, a in hits)
+        got = sorted((t * per_thread + n, a.hex()) for t, n, a in hits)
         assert got == [tuple(h) for h in fx["hits"]]
         assert b.checksum().hex() == fx["xor"]
     finally:
