@@ -42,10 +42,21 @@ METRIC = "keys/s on 1 MI355X vs plain HIP kernel; p50/p99 time-to-hit, mixed pre
 
 
 def mad_work_per_key(batch, affine=False):
-    """32x32->64 products per key on the 8x32-bit schoolbook count:
-    M(B) = 8 + 11/B multiplications and S(B) = 2 + 254/B squarings per key
-    for the projective walk; the affine sub-walk form needs 6 + 11/B."""
+    """32x32->64 products per key on the 8x32-bit schoolbook count of the
+    reference's walk: M(B) = 8 + 11/B multiplications and S(B) = 2 + 254/B
+    squarings per key for B keys per inversion (64 M + 36 S products). This
+    is the algorithmic work the roofline is priced in, whatever form computes
+    the keys; the affine sub-walk form's own count is 6 + 11/B."""
     return 64 * ((6 if affine else 8) + 11 / batch) + 36 * (2 + 254 / batch)
+
+
+def pair_mads_per_key(batch):
+    """Products per key the pair walk executes: per batch of B keys, B/2
+    forward and B backward multiplications of Montgomery's trick, 2M + 2S
+    per pair of keys (B/2 - 1 pairs), 1M + 1S for the single key, 2M + 1S for
+    the next centre and the inversion's 11M + 254S: M = 2.5 + 12/B and
+    S = 1 + 254/B."""
+    return 64 * (2.5 + 12 / batch) + 36 * (1 + 254 / batch)
 
 
 def measure_mad_peak():
@@ -174,6 +185,9 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--layout", type=int, default=lxv.LAYOUT_8X32)
+    ap.add_argument("--walk", choices=("pairs", "projective"), default="pairs",
+                    help="how a lane computes its batch (same keys): around an affine centre, or the reference's "
+                         "chained x-only differential additions")
     ap.add_argument("--subwalks", type=int, default=0,
                     help="K affine sub-walks per lane, inverted across the K (0: one walk per lane)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -190,8 +204,10 @@ def main():
 
     start, ed_seed, prefix = lxv.seeded_search(1, 7)
     ed_pub = lxv.ed25519_public(ed_seed)
+    walk = lxv.WALK_PAIRS if args.walk == "pairs" else lxv.WALK_PROJECTIVE
+    pairs = walk == lxv.WALK_PAIRS and not args.subwalks
     search = lxv.GpuSearch(start=start, ed_seed=ed_seed, batch=args.batch, waves_per_cu=args.waves_per_cu,
-                           layout=args.layout, subwalks=args.subwalks)
+                           layout=args.layout, subwalks=args.subwalks, walk=walk)
     b = search.backend
     per_iter = b.threads * b.batch
     budget = 1 << args.budget_log2
@@ -227,6 +243,8 @@ def main():
     roofline = {"bound": "valu-imad", "achieved": achieved / 1e12, "peak": (mad_peak or 0) / 1e12,
                 "unit": "T mad/s", "frac": (achieved / mad_peak) if mad_peak else None,
                 "traffic": None, "mads_per_key": work}
+    if pairs:
+        roofline["mads_per_key_executed"] = pair_mads_per_key(kpi)
     # HBM: the algorithmic bytes per key are the batch arrays (a_i and Z_i,
     # written in the forward pass and read back in the backward pass) plus
     # the lane state read and written once per launch; in the sub-walk form,
@@ -235,6 +253,10 @@ def main():
     fe_bytes = 4 * limbs
     if args.subwalks:
         alg_bytes_per_key = 2 * 2 * fe_bytes + 3 * 32
+    elif pairs:
+        # B/2 prefix products written and read back per batch of B keys, plus
+        # the centre (x, y) read and written once per launch
+        alg_bytes_per_key = 2 * fe_bytes / 2 + 2 * 2 * fe_bytes / (iters * b.batch)
     else:
         alg_bytes_per_key = 2 * 2 * fe_bytes + 2 * 4 * fe_bytes / (iters * b.batch)
     roofline["hbm"] = {"algorithmic_bytes_per_key": alg_bytes_per_key,
@@ -242,7 +264,8 @@ def main():
                        "frac": value * alg_bytes_per_key / 8e12}
     pmc = load_traffic()
     wpc = args.waves_per_cu or 16
-    kname = "%s<lxv::%s, %d, %d>" % ("walk_sw_kernel" if args.subwalks else "walk_kernel",
+    kname = "%s<lxv::%s, %d, %d>" % ("walk_sw_kernel" if args.subwalks else
+                                     ("walk_pair_kernel" if pairs else "walk_kernel"),
                                     "Field10" if args.layout == lxv.LAYOUT_10X26 else "Field8", kpi,
                                     4 if wpc >= 16 else (2 if wpc >= 8 else 1))
     if pmc and kname in pmc.get("kernel", "") and "hbm_bytes_per_key" in pmc:
@@ -264,8 +287,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": sum(kernel_ms) / len(kernel_ms), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u32 (exact integer limbs)",
         "data": "synthetic (seeded start scalar, Ed25519 key and prefix)",
-        "config": {"workload": ("plain HIP walk, single search, 7-hex prefix, B=%d, 2^%d-key budget"
-                                % (b.batch, args.budget_log2)) if not args.subwalks else
+        "config": {"workload": ("plain HIP walk (%s form), single search, 7-hex prefix, B=%d, 2^%d-key budget"
+                                % (args.walk, b.batch, args.budget_log2)) if not args.subwalks else
                                ("sub-walk form: %d affine walks per lane inverted together, single search, 7-hex "
                                 "prefix, 2^%d-key budget" % (args.subwalks, args.budget_log2)),
                    "threads": b.threads, "keys_per_inversion": kpi, "iters_per_step": iters,

@@ -47,6 +47,14 @@ extern "C" {
 #define LXV_LAYOUT_8X32 0   /* 8 x 32-bit limbs, reduction modulo 2^256 - 38 */
 #define LXV_LAYOUT_10X26 1  /* ref10 10-limb signed radix 2^25.5 */
 
+/* How a lane computes the public keys of its batch of B consecutive keys
+ * (the keys and their numbering are the same either way). */
+#define LXV_WALK_PAIRS 0       /* default: around an affine centre C, with C + iQ and
+                                  C - iQ sharing one inverse (Q = 8B): B/2 + 1
+                                  denominators per inversion, ~2.5 M + 1 S per key */
+#define LXV_WALK_PROJECTIVE 1  /* the reference's form: B chained x-only differential
+                                  additions in projective (X:Z), ~8 M + 2 S per key */
+
 typedef struct lxv_ctx lxv_ctx;
 
 /* Options for lxv_open. Zero-initialise, set `size`, override what you need. */
@@ -64,6 +72,7 @@ typedef struct lxv_open_opts {
                               of those walks (lxv_threads = lanes x K), lxv_batch is 1 (one
                               key per thread per iteration) and `threads` above counts
                               lanes */
+    uint32_t walk;         /* LXV_WALK_*, for subwalks = 0 */
 } lxv_open_opts;
 
 /* What one launch needs besides the per-lane state: LaunchArgs,
@@ -236,6 +245,9 @@ typedef struct lxv_multi_opts {
                                Smaller: more lanes per search, so a short search
                                finishes in fewer sequential blocks, at the cost of
                                one X25519 ladder per extra lane */
+    uint32_t walk;          /* LXV_WALK_*: PAIRS (default; 8x32, batch 64 or 256) keeps
+                               2 field elements of state per virtual lane,
+                               PROJECTIVE 4 */
 } lxv_multi_opts;
 
 typedef struct lxv_search_spec {

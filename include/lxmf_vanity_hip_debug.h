@@ -48,11 +48,21 @@ int lxv_debug_mad_rate(int device, double* mads_per_s, double* clock_hz, int* cu
 int lxv_debug_copy(int device, uint64_t bytes, double* ms);
 
 /* Register and scratch use of the walk kernel built for (layout, batch,
- * waves_per_cu): VGPRs per lane, scratch bytes per lane, and the waves per
- * SIMD its register budget was compiled for. Needs a GPU (reads the loaded
- * code object). */
-int lxv_debug_kernel_attrs(int layout, uint32_t batch, uint32_t waves_per_cu, uint32_t* vgprs, uint32_t* sgprs,
-                           uint32_t* scratch_bytes, uint32_t* waves_per_simd);
+ * waves_per_cu, walk form LXV_WALK_*): VGPRs per lane, scratch bytes per
+ * lane, and the waves per SIMD its register budget was compiled for. Needs a
+ * GPU (reads the loaded code object). */
+int lxv_debug_kernel_attrs(int layout, uint32_t batch, uint32_t waves_per_cu, uint32_t walk, uint32_t* vgprs,
+                           uint32_t* sgprs, uint32_t* scratch_bytes, uint32_t* waves_per_simd);
+
+/* The pair walk (LXV_WALK_PAIRS) of one lane, run by the host build of the
+ * device code: the lane's first centre from u_prev_cur (u(k-8), u(k) as 16
+ * little-endian words, the lxv_init form) or, when u_prev_cur is NULL, from
+ * the device ladder's set-up for the clamped scalar `start` (the
+ * lxv_init_seeded form); then `batches` batches of `batch` (4..256, a power
+ * of two) keys. Writes u(k + 8n) for n < batches x batch as 8 little-endian
+ * words each. CPU only. */
+int lxv_debug_pair_walk(int layout, uint32_t batch, const uint8_t start[32], const uint32_t* u_prev_cur,
+                        uint32_t batches, uint32_t* out);
 
 /* The command-line tool's CPU key walk (KeyWalk, src/main.rs:229-290, on
  * the host field arithmetic): the public keys u(start + 8n) for

@@ -1,4 +1,5 @@
-"""Autotuner of the walk kernel: keys per inversion x waves per CU x limb layout.
+"""Autotuner of the walk kernel: keys per inversion x waves per CU x limb layout
+(x walk form: the pair walk and the reference's projective walk).
 
 Each point opens the backend with that shape, sets every lane up on the GPU
 and walks a fixed budget of keys of one seeded search (an 8-hex-digit prefix
@@ -33,24 +34,25 @@ import lxv  # noqa: E402
 BATCHES = (16, 32, 64, 128, 256)
 WAVES = (2, 4, 8, 16)
 LAYOUTS = ((lxv.LAYOUT_8X32, "8x32"), (lxv.LAYOUT_10X26, "10x25.5"))
+WALKS = ((lxv.WALK_PAIRS, "pairs"), (lxv.WALK_PROJECTIVE, "projective"))
 SIMDS_PER_CU = 4
 XCDS = 8
 
 
-def kernel_attrs(layout, batch, wpc):
+def kernel_attrs(layout, batch, wpc, walk):
     L = lxv.lib()
-    L.lxv_debug_kernel_attrs.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32] + \
+    L.lxv_debug_kernel_attrs.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32] + \
         [ctypes.POINTER(ctypes.c_uint32)] * 4
     v, s, sc, w = (ctypes.c_uint32() for _ in range(4))
-    rc = L.lxv_debug_kernel_attrs(layout, batch, wpc, ctypes.byref(v), ctypes.byref(s), ctypes.byref(sc),
+    rc = L.lxv_debug_kernel_attrs(layout, batch, wpc, walk, ctypes.byref(v), ctypes.byref(s), ctypes.byref(sc),
                                   ctypes.byref(w))
     if rc != 0:
         return None
     return {"vgprs": v.value, "scratch_bytes": sc.value, "waves_per_simd": w.value}
 
 
-def run_point(layout, batch, wpc, budget, start, ed_seed, prefix):
-    s = lxv.GpuSearch(start=start, ed_seed=ed_seed, batch=batch, waves_per_cu=wpc, layout=layout)
+def run_point(layout, batch, wpc, walk, budget, start, ed_seed, prefix):
+    s = lxv.GpuSearch(start=start, ed_seed=ed_seed, batch=batch, waves_per_cu=wpc, layout=layout, walk=walk)
     try:
         b = s.backend
         per_iter = b.threads * b.batch
@@ -73,34 +75,37 @@ def run_point(layout, batch, wpc, budget, start, ed_seed, prefix):
         s.close()
 
 
-def sweep(budget):
+def sweep(budget, walks=WALKS):
     start, ed_seed, prefix = lxv.seeded_search(2, 8)
     points = []
-    for layout, lname in LAYOUTS:
-        for batch in BATCHES:
-            for wpc in WAVES:
-                t0 = time.time()
-                r = run_point(layout, batch, wpc, budget, start, ed_seed, prefix)
-                r.update({"layout": lname, "layout_id": layout, "batch": batch, "waves_per_cu": wpc})
-                attrs = kernel_attrs(layout, batch, wpc)
-                if attrs:
-                    r.update(attrs)
-                points.append(r)
-                print("%-8s B=%-3d waves/CU=%-2d  %.3e keys/s  vgprs=%s scratch=%s  (%.1f s)"
-                      % (lname, batch, wpc, r["keys_per_s"], r.get("vgprs"), r.get("scratch_bytes"),
-                         time.time() - t0), flush=True)
+    for walk, wname in walks:
+        for layout, lname in LAYOUTS:
+            for batch in BATCHES:
+                for wpc in WAVES:
+                    t0 = time.time()
+                    r = run_point(layout, batch, wpc, walk, budget, start, ed_seed, prefix)
+                    r.update({"walk": wname, "walk_id": walk, "layout": lname, "layout_id": layout, "batch": batch,
+                              "waves_per_cu": wpc})
+                    attrs = kernel_attrs(layout, batch, wpc, walk)
+                    if attrs:
+                        r.update(attrs)
+                    points.append(r)
+                    print("%-10s %-8s B=%-3d waves/CU=%-2d  %.3e keys/s  vgprs=%s scratch=%s  (%.1f s)"
+                          % (wname, lname, batch, wpc, r["keys_per_s"], r.get("vgprs"), r.get("scratch_bytes"),
+                             time.time() - t0), flush=True)
     return {"workload": "autotune sweep: single seeded search (seed 2), 8-hex prefix %s, 2^%d keys per point"
                         % (prefix, budget.bit_length() - 1),
             "points": points}
 
 
 def kernel_key(name, grid):
-    # walk_kernel<lxv::Field8, 64, 4> + lanes identifies a point
-    if "walk_kernel<" not in name:
-        return None
-    inner = name.split("walk_kernel<", 1)[1].split(">", 1)[0]
-    field, batch, waves = [x.strip() for x in inner.split(",")]
-    return ("8x32" if "Field8" in field else "10x25.5", int(batch), int(waves), int(grid))
+    # walk_pair_kernel<lxv::Field8, 64, 4> / walk_kernel<...> + lanes identifies a point
+    for kname, wname in (("walk_pair_kernel<", "pairs"), ("walk_kernel<", "projective")):
+        if kname in name:
+            inner = name.split(kname, 1)[1].split(">", 1)[0]
+            field, batch, waves = [x.strip() for x in inner.split(",")]
+            return (wname, "8x32" if "Field8" in field else "10x25.5", int(batch), int(waves), int(grid))
+    return None
 
 
 def merge_pmc(table, csv_path):
@@ -112,7 +117,7 @@ def merge_pmc(table, csv_path):
         d = acc.setdefault(k, {})
         d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
     for p in table["points"]:
-        k = (p["layout"], p["batch"], p.get("waves_per_simd", 0), p["lanes"])
+        k = (p.get("walk", "projective"), p["layout"], p["batch"], p.get("waves_per_simd", 0), p["lanes"])
         d = acc.get(k)
         if not d or not d.get("GRBM_GUI_ACTIVE"):
             continue
@@ -130,8 +135,10 @@ def pick(table):
     for p in table["points"]:
         if p["batch"] not in by_batch or p["keys_per_s"] > by_batch[p["batch"]]["keys_per_s"]:
             by_batch[p["batch"]] = p
-    table["pick"] = {k: best[k] for k in ("layout", "layout_id", "batch", "waves_per_cu", "keys_per_s")}
-    table["pick_by_batch"] = {str(b): {k: p[k] for k in ("layout", "layout_id", "waves_per_cu", "keys_per_s")}
+    table["pick"] = {k: best[k] for k in ("walk", "walk_id", "layout", "layout_id", "batch", "waves_per_cu",
+                                          "keys_per_s")}
+    table["pick_by_batch"] = {str(b): {k: p[k] for k in ("walk", "walk_id", "layout", "layout_id", "waves_per_cu",
+                                                         "keys_per_s")}
                               for b, p in sorted(by_batch.items())}
     return table
 
@@ -150,6 +157,7 @@ def main():
     ap.add_argument("--budget-log2", type=int, default=34)
     ap.add_argument("--out", default=os.path.join(HERE, "tuned.json"))
     ap.add_argument("--merge-pmc", default=None, help="PMC csv of a profiled pass to merge into --out")
+    ap.add_argument("--walk", choices=("all", "pairs", "projective"), default="all")
     args = ap.parse_args()
     if args.merge_pmc:
         table = load(args.out)
@@ -157,7 +165,8 @@ def main():
             raise SystemExit("no table at %s to merge into" % args.out)
         table = merge_pmc(table, args.merge_pmc)
     else:
-        table = pick(sweep(1 << args.budget_log2))
+        walks = [w for w in WALKS if args.walk in ("all", w[1])]
+        table = pick(sweep(1 << args.budget_log2, walks))
     if args.out != "/dev/null":
         with open(args.out, "w") as f:
             json.dump(table, f, indent=1)

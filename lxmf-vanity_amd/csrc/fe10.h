@@ -214,6 +214,14 @@ struct Field10 {
     LXV_HD static fe one() { return fe10_one(); }
     LXV_HD static fe add(const fe& a, const fe& b) { return fe10_add(a, b); }
     LXV_HD static fe sub(const fe& a, const fe& b) { return fe10_sub(a, b); }
+    // A sum of up to four tight values made tight again (for encoding or for
+    // further additions).
+    LXV_HD static fe tight(const fe& a) {
+        int64_t h[10];
+#pragma unroll
+        for (int i = 0; i < 10; i++) h[i] = a.v[i];
+        return fe10_carry(h);
+    }
     LXV_HD static fe mul(const fe& a, const fe& b) { return fe10_mul(a, b); }
     LXV_HD static fe sq(const fe& a) { return fe10_sq(a); }
     LXV_HD static fe invert(const fe& a) { return fe10_invert(a); }

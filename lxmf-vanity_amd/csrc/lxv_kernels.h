@@ -53,14 +53,30 @@ hipError_t launch_init_sw_ladder(int layout, uint32_t* state, uint64_t* ns, uint
                                  const uint32_t q[8], uint64_t stride, uint32_t lanes, uint32_t subwalks,
                                  hipStream_t s);
 
+// The pair walk (lxv_pairwalk.h, lxv_pair_kernels.hip): one walk per lane,
+// B keys per inversion around an affine centre; the state holds the centre
+// (x, y) in slots 0 and 1 of the projective walk's 4-element lane state.
+bool walk_pair_supported(int layout, int batch, int waves);
+hipError_t walk_pair_attributes(int layout, int batch, int waves, hipFuncAttributes* out);
+hipError_t launch_walk_pair(int layout, int batch, int waves, const WalkParams& p, hipStream_t s);
+hipError_t launch_init_pair_from_u(int layout, int batch, uint32_t* state, uint64_t* ns, uint32_t* xor_acc,
+                                   const uint32_t* upc, uint32_t lanes, hipStream_t s);
+// base = k_0 / 8 + (B/2 - 1) as 8 little-endian words; lane t starts at
+// k_0 + 8 t stride.
+hipError_t launch_init_pair_ladder(int layout, uint32_t* state, uint64_t* ns, uint32_t* xor_acc,
+                                   const uint32_t base[8], uint64_t stride, uint32_t lanes, hipStream_t s);
+
 struct MultiParams;
 struct MultiInitEntry;
-bool multi_supported(int layout, int batch);
-hipError_t launch_multi(int layout, int batch, const MultiParams& p, hipStream_t s);
+// walk: LXV_WALK_PROJECTIVE or LXV_WALK_PAIRS (pool state of 4 or 2 field
+// elements per virtual lane).
+bool multi_supported(int layout, int batch, int walk);
+hipError_t launch_multi(int layout, int batch, int walk, const MultiParams& p, hipStream_t s);
 // Sets up the virtual lanes of n admitted searches (entries in device memory)
 // in one launch; max_vlanes bounds every entry's lane count.
-hipError_t launch_multi_init(int layout, uint32_t* pool, uint32_t pool_lanes, const uint32_t q[8], uint64_t stride,
-                             const MultiInitEntry* entries, uint32_t n, uint32_t max_vlanes, hipStream_t s);
+hipError_t launch_multi_init(int layout, int walk, uint32_t* pool, uint32_t pool_lanes, const uint32_t q[8],
+                             uint64_t stride, const MultiInitEntry* entries, uint32_t n, uint32_t max_vlanes,
+                             hipStream_t s);
 hipError_t launch_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n,
                            hipStream_t s);
 // lanes must be a multiple of 256; 8 mads per lane per iteration.

@@ -129,6 +129,12 @@ __global__ __launch_bounds__(64, 4) void multi_walk_kernel(const MultiParams p) 
     if (plane < p.phys_lanes) multi_wave<F, B>(p, plane);
 }
 
+template <class F, int B>
+__global__ __launch_bounds__(64, 4) void multi_pair_kernel(const MultiParams p) {
+    const uint32_t plane = blockIdx.x * 64u + threadIdx.x;
+    if (plane < p.phys_lanes) multi_pair_wave<F, B>(p, plane);
+}
+
 struct MultiInitArgs {
     uint32_t q[8];
     uint64_t stride;
@@ -142,6 +148,13 @@ __global__ __launch_bounds__(64) void multi_init_kernel(uint32_t* pool, const Mu
     const MultiInitEntry& e = a.entries[blockIdx.y];
     const uint32_t v = blockIdx.x * 64u + threadIdx.x;
     if (v < e.vlanes) multi_init_lane<F>(pool, a.pool_lanes, e.state_off, e.base, a.stride, a.q, v);
+}
+
+template <class F>
+__global__ __launch_bounds__(64) void multi_pair_init_kernel(uint32_t* pool, const MultiInitArgs a) {
+    const MultiInitEntry& e = a.entries[blockIdx.y];
+    const uint32_t v = blockIdx.x * 64u + threadIdx.x;
+    if (v < e.vlanes) multi_pair_init_lane<F>(pool, a.pool_lanes, e.state_off, e.base, a.stride, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -287,23 +300,26 @@ typedef void (*multi_fn)(const MultiParams);
 struct MultiEntry {
     int layout;
     int batch;
+    int walk;
     multi_fn fn;
 };
 static const MultiEntry kMultis[] = {
-    {LXV_LAYOUT_8X32, 64, multi_walk_kernel<Field8, 64>},
-    {LXV_LAYOUT_8X32, 256, multi_walk_kernel<Field8, 256>},
-    {LXV_LAYOUT_10X26, 256, multi_walk_kernel<Field10, 256>},
+    {LXV_LAYOUT_8X32, 64, LXV_WALK_PROJECTIVE, multi_walk_kernel<Field8, 64>},
+    {LXV_LAYOUT_8X32, 256, LXV_WALK_PROJECTIVE, multi_walk_kernel<Field8, 256>},
+    {LXV_LAYOUT_10X26, 256, LXV_WALK_PROJECTIVE, multi_walk_kernel<Field10, 256>},
+    {LXV_LAYOUT_8X32, 64, LXV_WALK_PAIRS, multi_pair_kernel<Field8, 64>},
+    {LXV_LAYOUT_8X32, 256, LXV_WALK_PAIRS, multi_pair_kernel<Field8, 256>},
 };
 
-bool multi_supported(int layout, int batch) {
+bool multi_supported(int layout, int batch, int walk) {
     for (const auto& e : kMultis)
-        if (e.layout == layout && e.batch == batch) return true;
+        if (e.layout == layout && e.batch == batch && e.walk == walk) return true;
     return false;
 }
 
-hipError_t launch_multi(int layout, int batch, const MultiParams& p, hipStream_t s) {
+hipError_t launch_multi(int layout, int batch, int walk, const MultiParams& p, hipStream_t s) {
     for (const auto& e : kMultis) {
-        if (e.layout == layout && e.batch == batch) {
+        if (e.layout == layout && e.batch == batch && e.walk == walk) {
             hipLaunchKernelGGL(e.fn, dim3((p.phys_lanes + 63) / 64), dim3(64), 0, s, p);
             return hipGetLastError();
         }
@@ -311,8 +327,9 @@ hipError_t launch_multi(int layout, int batch, const MultiParams& p, hipStream_t
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_multi_init(int layout, uint32_t* pool, uint32_t pool_lanes, const uint32_t q[8], uint64_t stride,
-                             const MultiInitEntry* entries, uint32_t n, uint32_t max_vlanes, hipStream_t s) {
+hipError_t launch_multi_init(int layout, int walk, uint32_t* pool, uint32_t pool_lanes, const uint32_t q[8],
+                             uint64_t stride, const MultiInitEntry* entries, uint32_t n, uint32_t max_vlanes,
+                             hipStream_t s) {
     if (n == 0) return hipSuccess;
     if (n > 65535) return hipErrorInvalidValue;
     MultiInitArgs a;
@@ -321,7 +338,11 @@ hipError_t launch_multi_init(int layout, uint32_t* pool, uint32_t pool_lanes, co
     a.pool_lanes = pool_lanes;
     a.entries = entries;
     const dim3 grid((max_vlanes + 63) / 64, n), block(64);
-    if (layout == LXV_LAYOUT_8X32)
+    if (walk == LXV_WALK_PAIRS && layout == LXV_LAYOUT_8X32)
+        hipLaunchKernelGGL(multi_pair_init_kernel<Field8>, grid, block, 0, s, pool, a);
+    else if (walk == LXV_WALK_PAIRS)
+        return hipErrorInvalidValue;
+    else if (layout == LXV_LAYOUT_8X32)
         hipLaunchKernelGGL(multi_init_kernel<Field8>, grid, block, 0, s, pool, a);
     else
         hipLaunchKernelGGL(multi_init_kernel<Field10>, grid, block, 0, s, pool, a);

@@ -15,15 +15,22 @@
 // search's lowest hit block, never inside one, so every block up to the one
 // with the answer is walked completely and the answer does not depend on what
 // else runs.
+//
+// Two walk forms share the unit logic: the reference's projective walk
+// (multi_wave: four field elements X(n-1), Z(n-1), X(n), Z(n) of pool state
+// per virtual lane) and the pair walk of lxv_pairwalk.h (multi_pair_wave: the
+// affine batch centre (x, y), two field elements per virtual lane).
 #pragma once
 
+#include "lxv_pairwalk.h"
 #include "lxv_walk.h"
 
 namespace lxv {
 
 struct SlotConst {
     ShaSearch sha;
-    uint32_t base[8];     // start / 8 - 1, little-endian words (ladder setup)
+    uint32_t base[8];     // ladder set-up, little-endian words: start / 8 - 1
+                          // (projective) or start / 8 + B/2 - 1 (pair walk)
     uint64_t stride;      // key distance between virtual lanes
     uint32_t vlanes;      // V (a multiple of 64)
     uint32_t state_off;   // first pool lane of this search
@@ -39,7 +46,7 @@ struct SlotDyn {
 // One admitted search's virtual lanes, set up by the ladder in one batched
 // launch per step (block row = entry).
 struct MultiInitEntry {
-    uint32_t base[8];    // start / 8 - 1, little-endian words
+    uint32_t base[8];    // as SlotConst::base
     uint32_t state_off;  // first pool lane
     uint32_t vlanes;
     uint32_t pad[2];
@@ -68,7 +75,7 @@ struct MultiParams {
     const Unit* __restrict__ units;
     const uint32_t* n_units;   // device word written with the unit list
     uint32_t* counters;        // [0] next unit, [1] hit count, [2..3] keys (u64)
-    uint32_t* state;           // pool: [4 * kLimbs][pool_lanes]
+    uint32_t* state;           // pool: [4 * kLimbs][pool_lanes] (pair walk: [2 * kLimbs][pool_lanes])
     uint2* batch_mem;          // per physical lane
     MultiHit* hits;
     const uint32_t* stop;
@@ -79,6 +86,68 @@ struct MultiParams {
 
 LXV_D uint32_t wave_uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Hashes key n of virtual lane v (block blk) and records it if it matches:
+// one atomic per wave with matches, which also lowers the search's
+// lowest-hit block to blk.
+LXV_D void multi_emit(const MultiParams& p, const SlotConst& sc, uint32_t slot, uint32_t v, uint32_t blk, uint64_t n,
+                      const uint32_t uw[8]) {
+    const uint32_t me = __lane_id();
+    uint32_t addr[4];
+    lxmf_address_words(uw, sc.sha, addr);
+    const bool match = prefix_match(addr, sc.sha);
+    const uint64_t ball = __ballot(match);
+    if (ball) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(ball);
+        uint32_t base = 0;
+        if (me == leader) {
+            base = atomicAdd(&p.counters[1], (uint32_t)__builtin_popcountll(ball));
+            atomicMin(&p.dyn[slot].found_block, blk);
+        }
+        base = __shfl(base, (int)leader);
+        if (match) {
+            const uint32_t k = base + (uint32_t)__builtin_popcountll(ball & ((1ull << me) - 1ull));
+            if (k < p.max_hits) {
+                MultiHit h;
+                h.slot = slot;
+                h.vlane = v;
+                h.n = n;
+                for (int w = 0; w < 4; w++) h.address[w] = addr[w];
+                h.block = blk;
+                h.pad[0] = h.pad[1] = h.pad[2] = 0;
+                p.hits[k] = h;
+            }
+        }
+    }
+}
+
+// The next unit of the device queue for this wave (all lanes agree), or
+// false when the list is exhausted or a stop was asked for.
+LXV_D bool multi_next_unit(const MultiParams& p, Unit& unit) {
+    uint32_t u = 0;
+    if (__lane_id() == 0) u = atomicAdd(&p.counters[0], 1u);
+    u = wave_uniform(__shfl(u, 0));
+    if (u >= *p.n_units || stop_requested(p.stop)) return false;
+    unit.slot = wave_uniform(p.units[u].slot);
+    unit.vwave = wave_uniform(p.units[u].vwave);
+    unit.block0 = wave_uniform(p.units[u].block0);
+    unit.nblocks = wave_uniform(p.units[u].nblocks);
+    return true;
+}
+
+// Whether block blk may still be walked: not above the search's lowest block
+// with a hit.
+LXV_D bool multi_block_open(const MultiParams& p, uint32_t slot, uint32_t blk) {
+    const uint32_t fb = __hip_atomic_load(&p.dyn[slot].found_block, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return wave_uniform(fb) >= blk;
+}
+
+LXV_D void multi_count_keys(const MultiParams& p, unsigned long long keys) {
+    // keys walked by this lane, summed per wave then once into the counter
+    unsigned long long tot = keys;
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+    if (__lane_id() == 0 && tot) atomicAdd((unsigned long long*)&p.counters[2], tot);
+}
+
 template <class F, int B>
 LXV_D void multi_wave(const MultiParams& p, uint32_t plane) {
     typedef typename F::fe fe;
@@ -86,25 +155,16 @@ LXV_D void multi_wave(const MultiParams& p, uint32_t plane) {
     constexpr int kPairs = F::kLimbs / 2;
     const uint32_t PL = p.phys_lanes, SL = p.pool_lanes;
     const size_t slot_stride = (size_t)2 * kPairs * PL;
-    const uint32_t me = __lane_id();
     unsigned long long keys = 0;
-    for (;;) {
-        uint32_t u = 0;
-        if (me == 0) u = atomicAdd(&p.counters[0], 1u);
-        u = wave_uniform(__shfl(u, 0));
-        if (u >= *p.n_units || stop_requested(p.stop)) break;
-        const uint32_t slot = wave_uniform(p.units[u].slot);
-        const uint32_t vwave = wave_uniform(p.units[u].vwave);
-        const uint32_t block0 = wave_uniform(p.units[u].block0);
-        const uint32_t nblocks = wave_uniform(p.units[u].nblocks);
+    Unit unit;
+    while (multi_next_unit(p, unit)) {
+        const uint32_t slot = unit.slot;
         const SlotConst& sc = p.slots[slot];
         const uint32_t rounds = wave_uniform(sc.rounds);
-        const uint32_t v = vwave * 64u + me;
+        const uint32_t v = unit.vwave * 64u + __lane_id();
         const uint32_t sl = wave_uniform(sc.state_off) + v;
-        for (uint32_t blk = block0; blk < block0 + nblocks; blk++) {
-            const uint32_t fb =
-                __hip_atomic_load(&p.dyn[slot].found_block, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (wave_uniform(fb) < blk) break;
+        for (uint32_t blk = unit.block0; blk < unit.block0 + unit.nblocks; blk++) {
+            if (!multi_block_open(p, slot, blk)) break;
             for (uint32_t r = 0; r < rounds; r++) {
                 const uint64_t n0 = ((uint64_t)blk * rounds + r) * B;
                 fe prod;
@@ -134,42 +194,49 @@ LXV_D void multi_wave(const MultiParams& p, uint32_t plane) {
                     const uint2* bs = p.batch_mem + (size_t)i * slot_stride;
                     const fe u_ = F::mul(load_fe<F>(bs, PL, plane), inv);
                     if (i > 0) inv = F::mul(inv, load_fe<F>(bs + (size_t)kPairs * PL, PL, plane));
-                    uint32_t uw[8], addr[4];
+                    uint32_t uw[8];
                     F::to_words(u_, uw);
-                    lxmf_address_words(uw, sc.sha, addr);
-                    const bool match = prefix_match(addr, sc.sha);
-                    const uint64_t ball = __ballot(match);
-                    if (ball) {
-                        const uint32_t leader = (uint32_t)__builtin_ctzll(ball);
-                        uint32_t base = 0;
-                        if (me == leader) {
-                            base = atomicAdd(&p.counters[1], (uint32_t)__builtin_popcountll(ball));
-                            atomicMin(&p.dyn[slot].found_block, blk);
-                        }
-                        base = __shfl(base, (int)leader);
-                        if (match) {
-                            const uint32_t k = base + (uint32_t)__builtin_popcountll(ball & ((1ull << me) - 1ull));
-                            if (k < p.max_hits) {
-                                MultiHit h;
-                                h.slot = slot;
-                                h.vlane = v;
-                                h.n = n0 + (uint64_t)i;
-                                for (int w = 0; w < 4; w++) h.address[w] = addr[w];
-                                h.block = blk;
-                                h.pad[0] = h.pad[1] = h.pad[2] = 0;
-                                p.hits[k] = h;
-                            }
-                        }
-                    }
+                    multi_emit(p, sc, slot, v, blk, n0 + (uint64_t)i, uw);
                 }
                 keys += B;
             }
         }
     }
-    // keys walked by this lane, summed per wave then once into the counter
-    unsigned long long tot = keys;
-    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-    if (me == 0 && tot) atomicAdd((unsigned long long*)&p.counters[2], tot);
+    multi_count_keys(p, keys);
+}
+
+// The same units walked with the pair walk: the pool holds each virtual
+// lane's batch centre (x, y).
+template <class F, int B>
+LXV_D void multi_pair_wave(const MultiParams& p, uint32_t plane) {
+    typedef typename F::fe fe;
+    const uint32_t PL = p.phys_lanes, SL = p.pool_lanes;
+    unsigned long long keys = 0;
+    Unit unit;
+    while (multi_next_unit(p, unit)) {
+        const uint32_t slot = unit.slot;
+        const SlotConst& sc = p.slots[slot];
+        const uint32_t rounds = wave_uniform(sc.rounds);
+        const uint32_t v = unit.vwave * 64u + __lane_id();
+        const uint32_t sl = wave_uniform(sc.state_off) + v;
+        fe xc = load_state<F>(p.state, SL, sl, 0);
+        fe yc = load_state<F>(p.state, SL, sl, 1);
+        for (uint32_t blk = unit.block0; blk < unit.block0 + unit.nblocks; blk++) {
+            if (!multi_block_open(p, slot, blk)) break;
+            for (uint32_t r = 0; r < rounds; r++) {
+                const uint64_t n0 = ((uint64_t)blk * rounds + r) * B;
+                pair_batch<F, B>(xc, yc, p.batch_mem, PL, plane, [&](int i, const fe& x) {
+                    uint32_t uw[8];
+                    F::to_words(x, uw);
+                    multi_emit(p, sc, slot, v, blk, n0 + (uint64_t)i, uw);
+                });
+                keys += B;
+            }
+        }
+        store_state<F>(p.state, SL, sl, 0, xc);
+        store_state<F>(p.state, SL, sl, 1, yc);
+    }
+    multi_count_keys(p, keys);
 }
 
 // Virtual lanes [first, first + count) of one search set up by the ladder
@@ -184,6 +251,17 @@ LXV_D void multi_init_lane(uint32_t* pool, uint32_t pool_lanes, uint32_t state_o
     store_state<F>(pool, pool_lanes, lane, 1, z2);
     store_state<F>(pool, pool_lanes, lane, 2, x3);
     store_state<F>(pool, pool_lanes, lane, 3, z3);
+}
+
+// Virtual lane v of one search set up for the pair walk (base = start / 8 +
+// B/2 - 1): the centre of its first batch, with g = v * stride.
+template <class F>
+LXV_D void multi_pair_init_lane(uint32_t* pool, uint32_t pool_lanes, uint32_t state_off, const uint32_t base[8],
+                                uint64_t stride, uint32_t v) {
+    typename F::fe xc, yc;
+    pair_centre_ladder<F>(base, (uint64_t)v * stride, xc, yc);
+    store_state<F>(pool, pool_lanes, state_off + v, 0, xc);
+    store_state<F>(pool, pool_lanes, state_off + v, 1, yc);
 }
 
 }  // namespace lxv

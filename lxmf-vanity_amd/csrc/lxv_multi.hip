@@ -106,7 +106,7 @@ struct lxv_multi {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err = "no error";
-    int layout = LXV_LAYOUT_8X32, batch = 256;
+    int layout = LXV_LAYOUT_8X32, batch = 256, walk = LXV_WALK_PAIRS;
     uint32_t rounds = 1, limbs = 8;
     uint32_t phys_lanes = 0, pool_lanes = 0, max_slots = 1024, max_vlanes = 1u << 18, lane_keys_log2 = 14;
     bool use_graph = true;
@@ -224,6 +224,7 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
     m->device = o.device < 0 ? 0 : o.device;
     m->batch = o.batch ? (int)o.batch : 256;
     m->layout = (int)o.layout;
+    m->walk = (int)o.walk;
     m->use_graph = !o.no_graph;
     if (o.max_searches) m->max_slots = o.max_searches;
     if (o.max_vlanes) m->max_vlanes = std::max<uint32_t>(64, o.max_vlanes / 64 * 64);
@@ -234,9 +235,9 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
         }
         m->lane_keys_log2 = o.lane_keys_log2;
     }
-    if (!multi_supported(m->layout, m->batch)) {
+    if (!multi_supported(m->layout, m->batch, m->walk)) {
         delete m;
-        return mfail(nullptr, LXV_E_ARG, "lxv_multi_open: no batched kernel for this layout and batch");
+        return mfail(nullptr, LXV_E_ARG, "lxv_multi_open: no batched kernel for this layout, batch and walk");
     }
     int ndev = 0;
     hipDeviceProp_t prop;
@@ -248,7 +249,9 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
     m->limbs = (uint32_t)limbs_of(m->layout);
     m->phys_lanes = (uint32_t)prop.multiProcessorCount * 16u * 64u;
     const uint64_t pool_bytes = o.pool_bytes ? o.pool_bytes : (2ull << 30);
-    const uint64_t lane_bytes = 4ull * m->limbs * sizeof(uint32_t);
+    // pool state per virtual lane: the pair walk's centre (x, y), or the
+    // projective walk's X(n-1), Z(n-1), X(n), Z(n)
+    const uint64_t lane_bytes = (m->walk == LXV_WALK_PAIRS ? 2ull : 4ull) * m->limbs * sizeof(uint32_t);
     m->pool_lanes = (uint32_t)std::min<uint64_t>(pool_bytes / lane_bytes / 64 * 64, 0xffffffc0ull);
     if (m->pool_lanes < 64) {
         delete m;
@@ -277,7 +280,8 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
     MALLOC(m->d_nunits, 64);
     MALLOC(m->d_counters, 64);
     MALLOC(m->d_pool, (size_t)m->pool_lanes * lane_bytes);
-    MALLOC(m->d_batch, (size_t)m->phys_lanes * m->batch * 2 * m->limbs * sizeof(uint32_t));
+    MALLOC(m->d_batch, (size_t)m->phys_lanes * (m->walk == LXV_WALK_PAIRS ? m->batch / 2 : m->batch * 2) * m->limbs *
+                           sizeof(uint32_t));
     MALLOC(m->d_hits, (size_t)kMaxHits * sizeof(MultiHit));
     MALLOC(m->d_init, (size_t)m->max_slots * sizeof(MultiInitEntry));
 #undef MALLOC
@@ -344,7 +348,8 @@ extern "C" int lxv_multi_submit(lxv_multi* m, const lxv_search_spec* spec, uint6
     nh[2] = (uint32_t)nhb[8] << 24 | (uint32_t)nhb[9] << 16;
     std::memset(&s.sc, 0, sizeof s.sc);
     sha_search_init(s.sc.sha, ed, nh, want, mask);
-    // base = start / 8 - 1
+    // base = start / 8 - 1 (the projective walk's u(k - 8)), or start / 8 +
+    // B/2 - 1 (the pair walk's first centre); no wrap for a start scalar
     uint8_t s8[32];
     uint32_t carry = 0;
     for (int i = 31; i >= 0; i--) {
@@ -354,8 +359,17 @@ extern "C" int lxv_multi_submit(lxv_multi* m, const lxv_search_spec* spec, uint6
     for (int i = 0; i < 8; i++)
         s.sc.base[i] = (uint32_t)s8[4 * i] | (uint32_t)s8[4 * i + 1] << 8 | (uint32_t)s8[4 * i + 2] << 16 |
                        (uint32_t)s8[4 * i + 3] << 24;
-    for (int i = 0; i < 8; i++)
-        if (s.sc.base[i]-- != 0) break;
+    if (m->walk == LXV_WALK_PAIRS) {
+        uint64_t c = (uint64_t)(m->batch / 2 - 1);
+        for (int i = 0; i < 8 && c; i++) {
+            const uint64_t v = (uint64_t)s.sc.base[i] + c;
+            s.sc.base[i] = (uint32_t)v;
+            c = v >> 32;
+        }
+    } else {
+        for (int i = 0; i < 8; i++)
+            if (s.sc.base[i]-- != 0) break;
+    }
     s.sc.stride = kStride;
     s.sc.vlanes = s.vlanes;
     s.sc.rounds = m->rounds;
@@ -439,7 +453,7 @@ static int admit(lxv_multi* m) {
     if (n) {
         MCHK(m, hipMemcpyAsync(m->d_init, m->h_init, (size_t)n * sizeof(MultiInitEntry), hipMemcpyHostToDevice,
                                m->stream));
-        MCHK(m, launch_multi_init(m->layout, m->d_pool, m->pool_lanes, m->q, kStride, m->d_init, n, max_vl,
+        MCHK(m, launch_multi_init(m->layout, m->walk, m->d_pool, m->pool_lanes, m->q, kStride, m->d_init, n, max_vl,
                                   m->stream));
         MCHK(m, hipEventRecord(m->ev_admit, m->stream));
         m->admit_pending = true;
@@ -452,7 +466,7 @@ static int launch_step(lxv_multi* m) {
         if (!m->gexec) {
             MCHK(m, hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
             hipError_t e = launch_clear(m->d_counters, 16, m->stream);
-            if (e == hipSuccess) e = launch_multi(m->layout, m->batch, m->params, m->stream);
+            if (e == hipSuccess) e = launch_multi(m->layout, m->batch, m->walk, m->params, m->stream);
             hipError_t e2 = hipStreamEndCapture(m->stream, &m->graph);
             if (e != hipSuccess) return mfail(m, LXV_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
             MCHK(m, e2);
@@ -464,7 +478,7 @@ static int launch_step(lxv_multi* m) {
     } else {
         MCHK(m, launch_clear(m->d_counters, 16, m->stream));
         MCHK(m, hipEventRecord(m->ev0, m->stream));
-        MCHK(m, launch_multi(m->layout, m->batch, m->params, m->stream));
+        MCHK(m, launch_multi(m->layout, m->batch, m->walk, m->params, m->stream));
         MCHK(m, hipEventRecord(m->ev1, m->stream));
     }
     return LXV_OK;

@@ -1,0 +1,231 @@
+// The pair walk: one walk per lane over consecutive keys k + 8n, as the
+// reference's walk (src/gpu/common.h:280-365, src/main.rs:255-289), with the
+// public keys of a batch of B keys computed around an affine centre instead
+// of by B chained differential additions.
+//
+// A lane holds the affine point C = (x, y) = (k/8 + c) Q of its batch centre,
+// key index c = n + H with H = B/2 - 1 (Q = 8B; T_i = i Q from
+// pair_table.inc). The batch's keys are
+//
+//     c                       x(C)
+//     c + i, c - i (i <= H)   x(C + T_i), x(C - T_i)
+//     c + H + 1 = n + B - 1   x(C + T_{H+1})
+//
+// and the next centre is C + T_B. For T = (u, v) the affine sums are
+//
+//     x(C +- T) = l^2 - A - x - u,   l = (v -+ y) / (u - x)
+//
+// (only l^2 is needed, so the sign of l for C - T is free), so C + T_i and
+// C - T_i share the inverse of u_i - x. The B/2 + 1 denominators of a batch
+// are inverted together with Montgomery's trick: per key about 2.5 M + 1 S
+// plus the inversion's share, against 8 M + 2 S for the projective walk of
+// lxv_walk.h at the same B keys per inversion.
+//
+// Denominators are numbered e = 0 (the centre step, T_B) and e = 1 .. H + 1
+// (T_e). The forward pass stores the prefix products d_0 ... d_{e-1} for
+// e >= 1 (B/2 field elements per lane, lane-interleaved in HBM as the
+// projective walk's batch arrays); the backward pass walks e downwards,
+// two keys per pair.
+#pragma once
+
+#include "lxv_walk.h"
+#include "pair_table.inc"
+
+namespace lxv {
+
+#if defined(__HIP_DEVICE_COMPILE__)
+static __constant__ uint32_t kPairTab[LXV_PAIR_TABLE_ENTRIES * 16] = LXV_PAIR_TABLE_INIT;
+#else
+static const uint32_t kPairTab[LXV_PAIR_TABLE_ENTRIES * 16] = LXV_PAIR_TABLE_INIT;
+#endif
+
+constexpr uint32_t kMontA = 486662;  // Curve25519: v^2 = u^3 + A u^2 + u
+
+LXV_HD int pair_tab_entry(int i) { return i <= 128 ? i - 1 : 128; }
+
+// u and v of T_i (1 <= i <= 128, or i = 256) in the layout of F.
+template <class F>
+LXV_HD typename F::fe pair_tab_u(int i) {
+    return F::from_words(kPairTab + 16 * pair_tab_entry(i));
+}
+template <class F>
+LXV_HD typename F::fe pair_tab_v(int i) {
+    return F::from_words(kPairTab + 16 * pair_tab_entry(i) + 8);
+}
+
+template <class F>
+LXV_HD typename F::fe fe_small(uint32_t k) {
+    const uint32_t w[8] = {k, 0, 0, 0, 0, 0, 0, 0};
+    return F::from_words(w);
+}
+
+// One batch of B keys around the centre (xc, yc), which moves on to the next
+// batch's centre. emit(i, x) receives the public key of key n + i (i < B) as
+// a field element; mem is the lane's prefix-product array (B/2 elements,
+// lane-interleaved with stride lanes).
+template <class F, int B, class Emit>
+LXV_HD void pair_batch(typename F::fe& xc, typename F::fe& yc, uint2* mem, uint32_t lanes, uint32_t lane,
+                       Emit&& emit) {
+    typedef typename F::fe fe;
+    static_assert(B >= 4 && B <= 256 && (B & (B - 1)) == 0, "B is a power of two in [4, 256]");
+    constexpr int H = B / 2 - 1;
+    constexpr int E = B / 2 + 1;
+    constexpr int kPairs = F::kLimbs / 2;
+    const size_t slot = (size_t)kPairs * lanes;
+
+    // forward: d_0 = u(T_B) - x, then d_e = u(T_e) - x
+    fe prod = F::sub(pair_tab_u<F>(B), xc);
+    for (int e = 1; e < E; e++) {
+        store_fe<F>(mem + (size_t)(e - 1) * slot, lanes, lane, prod);
+        prod = F::mul(prod, F::sub(pair_tab_u<F>(e), xc));
+    }
+    fe inv = F::invert(prod);
+    const fe sa = F::tight(F::add(xc, fe_small<F>(kMontA)));
+
+    // backward, one key per step: j = 0 the single key (e = H + 1), then
+    // e = H .. 1 with C + T_e on odd j and C - T_e on even j, then the
+    // centre's own key. The element's inverse is formed on the first key.
+    fe ie, s, ve;
+    int e = H + 1;
+    for (int j = 0; j < B; j++) {
+        fe x;
+        int i;
+        if (j == B - 1) {
+            x = xc;
+            i = H;
+        } else if (j == 0 || (j & 1)) {
+            e = j == 0 ? H + 1 : H - (j - 1) / 2;
+            const fe ue = pair_tab_u<F>(e);
+            ve = pair_tab_v<F>(e);
+            ie = F::mul(inv, load_fe<F>(mem + (size_t)(e - 1) * slot, lanes, lane));
+            inv = F::mul(inv, F::sub(ue, xc));
+            s = F::tight(F::add(sa, ue));
+            x = F::tight(F::sub(F::sq(F::mul(F::sub(ve, yc), ie)), s));
+            i = H + e;
+        } else {
+            x = F::tight(F::sub(F::sq(F::mul(F::add(ve, yc), ie)), s));
+            i = H - e;
+        }
+        emit(i, x);
+    }
+
+    // the next centre, C + T_B: inv is now 1 / d_0
+    const fe ub = pair_tab_u<F>(B), vb = pair_tab_v<F>(B);
+    const fe l = F::mul(F::sub(vb, yc), inv);
+    const fe x2 = F::tight(F::sub(F::sq(l), F::tight(F::add(sa, ub))));
+    yc = F::tight(F::sub(F::mul(l, F::sub(xc, x2)), yc));
+    xc = x2;
+}
+
+// y of R from x(R) = xr / zr and x(R + T) = xs / zs for T = (ut, vt) with
+// vt != 0 (Okeya-Sakurai recovery: the affine sum formula solved for y(R)),
+//     2 vt y = (ut x + 1)(ut + x + 2A) - 2A - x(R + T) (x - ut)^2,
+// returned as the affine pair (x, y) with one inversion.
+template <class F>
+LXV_HD void recover_affine(const typename F::fe& xr, const typename F::fe& zr, const typename F::fe& xs,
+                           const typename F::fe& zs, const typename F::fe& ut, const typename F::fe& vt,
+                           typename F::fe& x, typename F::fe& y) {
+    typedef typename F::fe fe;
+    const fe a2 = fe_small<F>(2 * kMontA);
+    const fe uz = F::mul(ut, zr);
+    const fe t1 = F::add(F::mul(ut, xr), zr);                  // (ut x + 1) zr
+    const fe t2 = F::tight(F::add(F::add(uz, xr), F::mul(a2, zr)));  // (ut + x + 2A) zr
+    const fe zz = F::mul(F::sq(zr), zs);                       // zr^2 zs
+    const fe d = F::sub(xr, uz);                               // (x - ut) zr
+    const fe num = F::tight(F::sub(F::sub(F::mul(F::mul(t1, t2), zs), F::mul(a2, zz)), F::mul(xs, F::sq(d))));
+    const fe v2 = F::tight(F::add(vt, vt));
+    const fe den = F::mul(v2, zz);                             // 2 vt zr^2 zs
+    const fe inv = F::invert(den);
+    y = F::mul(num, inv);
+    x = F::mul(F::mul(F::mul(xr, zr), F::mul(zs, v2)), inv);
+}
+
+// (x, y) + T_i by the affine addition formula (x != u(T_i)).
+template <class F>
+LXV_HD void affine_add_tab(typename F::fe& x, typename F::fe& y, int i) {
+    typedef typename F::fe fe;
+    const fe ut = pair_tab_u<F>(i), vt = pair_tab_v<F>(i);
+    const fe l = F::mul(F::sub(vt, y), F::invert(F::sub(ut, x)));
+    const fe x2 = F::tight(F::sub(F::sub(F::sq(l), fe_small<F>(kMontA)), F::add(x, ut)));
+    y = F::tight(F::sub(F::mul(l, F::sub(x, x2)), y));
+    x = x2;
+}
+
+// The centre of a lane's first batch from the reference's start data: u(k - 8)
+// and u(k) (8 little-endian words each). R = kB = (k/8) Q has R - Q = u(k - 8),
+// so y(R) follows from the recovery with T = -Q; the centre is R + T_H.
+template <class F, int B>
+LXV_HD void pair_centre_from_u(const uint32_t uprev[8], const uint32_t ucur[8], typename F::fe& xc,
+                               typename F::fe& yc) {
+    typedef typename F::fe fe;
+    const fe uq = pair_tab_u<F>(1), vq = pair_tab_v<F>(1);
+    recover_affine<F>(F::from_words(ucur), F::one(), F::from_words(uprev), F::one(), uq, F::sub(F::zero(), vq), xc,
+                      yc);
+    affine_add_tab<F>(xc, yc, B / 2 - 1);
+}
+
+// The centre of a lane's first batch by a ladder: m = base + off with
+// base = k_0 / 8 + H; the ladder over Q gives mQ and (m + 1)Q projectively,
+// and the recovery with T = Q makes mQ affine with its y.
+template <class F>
+LXV_HD void pair_centre_ladder(const uint32_t base[8], uint64_t off, typename F::fe& xc, typename F::fe& yc) {
+    typedef typename F::fe fe;
+    uint32_t qw[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) qw[k] = kPairTab[k];
+    fe x2, z2, x3, z3;
+    ladder_mq<F>(base, off, qw, x2, z2, x3, z3);
+    recover_affine<F>(x2, z2, x3, z3, pair_tab_u<F>(1), pair_tab_v<F>(1), xc, yc);
+}
+
+#if defined(__HIPCC__)
+// The walk of one lane: `iters` batches of B keys. The centre is the lane's
+// whole state (state slots 0 and 1: x and y).
+template <class F, int B>
+__device__ __forceinline__ void walk_pair_lane(const WalkParams& p, uint32_t lane) {
+    typedef typename F::fe fe;
+    const uint32_t L = p.lanes;
+    const uint64_t n0 = p.ns[lane];
+    uint64_t n = n0;
+    uint32_t x0 = p.xor_acc[0 * L + lane], x1 = p.xor_acc[1 * L + lane];
+    uint32_t x2 = p.xor_acc[2 * L + lane], x3 = p.xor_acc[3 * L + lane];
+    fe xc = load_state<F>(p.state, L, lane, 0);
+    fe yc = load_state<F>(p.state, L, lane, 1);
+
+    for (uint32_t it = 0; it < p.iters; it++) {
+        if (stop_requested(p.stop)) {
+            *p.stopped = 1u;
+            break;
+        }
+        const bool dumping = it == 0 && lane < p.dump_threads;
+        pair_batch<F, B>(xc, yc, p.batch_mem, L, lane, [&](int i, const fe& x) {
+            uint32_t uw[8];
+            F::to_words(x, uw);
+            if (dumping) {
+                uint32_t* d = p.dump + ((size_t)lane * B + i) * 8;
+#pragma unroll
+                for (int k = 0; k < 8; k++) d[k] = uw[k];
+            }
+            uint32_t addr[4];
+            lxmf_address_words(uw, p.sha, addr);
+            x0 ^= addr[0];
+            x1 ^= addr[1];
+            x2 ^= addr[2];
+            x3 ^= addr[3];
+            append_hit(prefix_match(addr, p.sha), lane, n + (uint64_t)i, addr, p.hits, p.hit_count, p.max_hits);
+        });
+        n += B;
+    }
+
+    store_state<F>(p.state, L, lane, 0, xc);
+    store_state<F>(p.state, L, lane, 1, yc);
+    p.ns[lane] = n;
+    p.xor_acc[0 * L + lane] = x0;
+    p.xor_acc[1 * L + lane] = x1;
+    p.xor_acc[2 * L + lane] = x2;
+    p.xor_acc[3 * L + lane] = x3;
+    atomicAdd(p.batches, (unsigned long long)((n - n0) / B));
+}
+#endif
+
+}  // namespace lxv

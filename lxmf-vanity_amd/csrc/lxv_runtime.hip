@@ -14,6 +14,7 @@
 #include "fe8.h"
 #include "host_crypto.h"
 #include "lxv_kernels.h"
+#include "lxv_pairwalk.h"
 #include "lxv_walk.h"
 
 using namespace lxv;
@@ -36,7 +37,8 @@ struct lxv_ctx {
     std::string desc;
     std::string err = "no error";
     uint32_t lanes = 0, batch = 64, waves = 4, layout = LXV_LAYOUT_10X26, limbs = 10;
-    uint32_t subwalks = 0;  // 0: one projective walk per lane; K: K affine sub-walks per lane
+    uint32_t subwalks = 0;  // 0: one walk per lane; K: K affine sub-walks per lane
+    uint32_t walk = LXV_WALK_PAIRS;  // the form of the one-walk-per-lane kernels
     uint32_t phys = 0;      // physical lanes (= lanes unless sub-walks)
     uint32_t* d_state = nullptr;
     uint64_t* d_ns = nullptr;
@@ -84,10 +86,10 @@ static void free_ctx(lxv_ctx* c) {
     (void)hipFree(c->d_counters);
     (void)hipFree(c->d_hits);
     (void)hipFree(c->d_dump);
-    if (c->h_stop) hipHostFree(c->h_stop);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->h_stop) (void)hipHostFree(c->h_stop);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -135,8 +137,14 @@ extern "C" int lxv_open(const lxv_open_opts* opts, lxv_ctx** out) {
         c->batch = 1;
     } else {
         c->batch = o.batch ? o.batch : 64;
+        c->walk = o.walk;
+        if (c->walk != LXV_WALK_PAIRS && c->walk != LXV_WALK_PROJECTIVE) {
+            delete c;
+            return fail(nullptr, LXV_E_ARG, "lxv_open: unknown walk form");
+        }
     }
-    if (!c->subwalks && !walk_supported((int)c->layout, (int)c->batch, (int)c->waves)) {
+    if (!c->subwalks && !(c->walk == LXV_WALK_PAIRS ? walk_pair_supported((int)c->layout, (int)c->batch, (int)c->waves)
+                                                    : walk_supported((int)c->layout, (int)c->batch, (int)c->waves))) {
         delete c;
         return fail(nullptr, LXV_E_ARG,
                     "lxv_open: no walk kernel built for this layout / batch / waves (batch must be 16..256, a power of two)");
@@ -168,6 +176,10 @@ extern "C" int lxv_open(const lxv_open_opts* opts, lxv_ctx** out) {
         // two slots of 8 canonical words per sub-walk; a_j and D_j per sub-walk
         ALLOC(c->d_state, L * c->subwalks * 2 * 8 * sizeof(uint32_t));
         ALLOC(c->d_batch, L * (size_t)c->subwalks * 2 * c->limbs * sizeof(uint32_t));
+    } else if (c->walk == LXV_WALK_PAIRS) {
+        // the centre (x, y) in slots 0-1 of the lane state; B/2 prefix products
+        ALLOC(c->d_state, L * 4 * c->limbs * sizeof(uint32_t));
+        ALLOC(c->d_batch, L * (size_t)(c->batch / 2) * c->limbs * sizeof(uint32_t));
     } else {
         ALLOC(c->d_state, L * 4 * c->limbs * sizeof(uint32_t));
         ALLOC(c->d_batch, L * (size_t)c->batch * 2 * c->limbs * sizeof(uint32_t));
@@ -211,16 +223,43 @@ extern "C" int lxv_init(lxv_ctx* c, const uint32_t* upc, size_t n_words) {
     uint32_t* d_upc = nullptr;
     HIPCHK(c, hipMalloc((void**)&d_upc, n_words * sizeof(uint32_t)));
     hipError_t e = hipMemcpyAsync(d_upc, upc, n_words * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess)
-        e = c->subwalks ? launch_init_sw_from_u((int)c->layout, c->d_state, c->d_ns, c->d_xor, d_upc, c->phys,
-                                                c->subwalks, c->stream)
-                        : launch_init_from_u((int)c->layout, c->d_state, c->d_ns, c->d_xor, d_upc, c->lanes,
-                                             c->stream);
+    if (e == hipSuccess) {
+        if (c->subwalks)
+            e = launch_init_sw_from_u((int)c->layout, c->d_state, c->d_ns, c->d_xor, d_upc, c->phys, c->subwalks,
+                                      c->stream);
+        else if (c->walk == LXV_WALK_PAIRS)
+            e = launch_init_pair_from_u((int)c->layout, (int)c->batch, c->d_state, c->d_ns, c->d_xor, d_upc,
+                                        c->lanes, c->stream);
+        else
+            e = launch_init_from_u((int)c->layout, c->d_state, c->d_ns, c->d_xor, d_upc, c->lanes, c->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d_upc);
     if (e != hipSuccess) return hip_fail(c, e, "lxv_init");
     c->initialized = true;
     return LXV_OK;
+}
+
+// k / 8 + d as 8 little-endian words, for a multiple of 8 with bit 254 set
+// (start scalars) and |d| < 2^31, so nothing wraps.
+static void scalar_div8_plus(const uint8_t k[32], int64_t d, uint32_t out[8]) {
+    uint8_t s8[32];
+    uint32_t carry = 0;
+    for (int i = 31; i >= 0; i--) {
+        s8[i] = (uint8_t)((k[i] >> 3) | (carry << 5));
+        carry = k[i] & 7;
+    }
+    uint64_t acc = (uint64_t)(int64_t)d;  // two's complement, sign-extended through every word
+    const uint64_t ext = d < 0 ? 0xffffffffull : 0;
+    uint64_t c = 0;
+    for (int i = 0; i < 8; i++) {
+        const uint64_t w = (uint64_t)s8[4 * i] | (uint64_t)s8[4 * i + 1] << 8 | (uint64_t)s8[4 * i + 2] << 16 |
+                           (uint64_t)s8[4 * i + 3] << 24;
+        const uint64_t add = i == 0 ? (acc & 0xffffffffull) : (i == 1 ? (acc >> 32) : ext);
+        const uint64_t v = w + add + c;
+        out[i] = (uint32_t)v;
+        c = v >> 32;
+    }
 }
 
 extern "C" int lxv_init_seeded(lxv_ctx* c, const uint8_t start[32], uint64_t stride) {
@@ -229,27 +268,18 @@ extern "C" int lxv_init_seeded(lxv_ctx* c, const uint8_t start[32], uint64_t str
     if (!start || !start_scalar_ok(start)) return fail(c, LXV_E_ARG, "lxv_init_seeded: not a valid start scalar");
     if (stride == 0 || (c->lanes > 1 && stride > (UINT64_MAX >> 1) / c->lanes))
         return fail(c, LXV_E_ARG, "lxv_init_seeded: lanes x stride must stay below 2^63");
-    // base = start / 8 - 1 as little-endian words
-    uint8_t s8[32];
-    uint32_t carry = 0;
-    for (int i = 31; i >= 0; i--) {
-        s8[i] = (uint8_t)((start[i] >> 3) | (carry << 5));
-        carry = start[i] & 7;
-    }
+    // base = start / 8 - 1 (the ladder gives u(k - 8), u(k)), or for the pair
+    // walk start / 8 + B/2 - 1 (the first batch's centre)
     uint32_t base[8];
-    for (int i = 0; i < 8; i++)
-        base[i] = (uint32_t)s8[4 * i] | (uint32_t)s8[4 * i + 1] << 8 | (uint32_t)s8[4 * i + 2] << 16 |
-                  (uint32_t)s8[4 * i + 3] << 24;
-    uint64_t borrow = 1;
-    for (int i = 0; i < 8 && borrow; i++) {
-        borrow = base[i] == 0;
-        base[i] -= 1;
-    }
+    scalar_div8_plus(start, c->walk == LXV_WALK_PAIRS && !c->subwalks ? (int64_t)c->batch / 2 - 1 : -1, base);
     lxv_launch_args a{};
     uint8_t zero[32] = {0};
     lxv_launch_args_for(zero, &a);
     HIPCHK(c, hipSetDevice(c->device));
-    if (c->subwalks)
+    if (!c->subwalks && c->walk == LXV_WALK_PAIRS)
+        HIPCHK(c, launch_init_pair_ladder((int)c->layout, c->d_state, c->d_ns, c->d_xor, base, stride, c->lanes,
+                                          c->stream));
+    else if (c->subwalks)
         HIPCHK(c, launch_init_sw_ladder((int)c->layout, c->d_state, c->d_ns, c->d_xor, base, a.q, stride, c->phys,
                                         c->subwalks, c->stream));
     else
@@ -327,6 +357,8 @@ extern "C" int lxv_launch(lxv_ctx* c, const lxv_launch_args* a) {
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (c->subwalks)
         HIPCHK(c, launch_walk_sw((int)c->layout, (int)c->subwalks, (int)c->waves, p, c->stream));
+    else if (c->walk == LXV_WALK_PAIRS)
+        HIPCHK(c, launch_walk_pair((int)c->layout, (int)c->batch, (int)c->waves, p, c->stream));
     else
         HIPCHK(c, launch_walk((int)c->layout, (int)c->batch, (int)c->waves, p, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -589,15 +621,70 @@ extern "C" int lxv_debug_copy(int device, uint64_t bytes, double* ms) {
     return e == hipSuccess ? LXV_OK : LXV_E_HIP;
 }
 
-extern "C" int lxv_debug_kernel_attrs(int layout, uint32_t batch, uint32_t waves_per_cu, uint32_t* vgprs,
-                                      uint32_t* sgprs, uint32_t* scratch_bytes, uint32_t* waves_per_simd) {
+extern "C" int lxv_debug_kernel_attrs(int layout, uint32_t batch, uint32_t waves_per_cu, uint32_t walk,
+                                      uint32_t* vgprs, uint32_t* sgprs, uint32_t* scratch_bytes,
+                                      uint32_t* waves_per_simd) {
     const int w = waves_per_cu >= 16 ? 4 : (waves_per_cu >= 8 ? 2 : 1);
-    if (!walk_supported(layout, (int)batch, w)) return LXV_E_ARG;
+    const bool pairs = walk == LXV_WALK_PAIRS;
+    if (pairs ? !walk_pair_supported(layout, (int)batch, w) : !walk_supported(layout, (int)batch, w))
+        return LXV_E_ARG;
     hipFuncAttributes a;
-    if (walk_attributes(layout, (int)batch, w, &a) != hipSuccess) return LXV_E_HIP;
+    if ((pairs ? walk_pair_attributes(layout, (int)batch, w, &a) : walk_attributes(layout, (int)batch, w, &a)) !=
+        hipSuccess)
+        return LXV_E_HIP;
     if (vgprs) *vgprs = (uint32_t)a.numRegs;
     if (sgprs) *sgprs = 0;
     if (scratch_bytes) *scratch_bytes = (uint32_t)a.localSizeBytes;
     if (waves_per_simd) *waves_per_simd = (uint32_t)w;
     return LXV_OK;
+}
+
+namespace lxv {
+
+// The host build of the same walk, for CPU checks (lxv_debug_pair_walk).
+template <class F, int B>
+static void host_pair_walk(const uint32_t* upc, const uint32_t base[8], uint32_t batches, uint32_t* out) {
+    typename F::fe xc, yc;
+    if (upc)
+        pair_centre_from_u<F, B>(upc, upc + 8, xc, yc);
+    else
+        pair_centre_ladder<F>(base, 0, xc, yc);
+    uint2 mem[(B / 2) * (F::kLimbs / 2)];
+    for (uint32_t b = 0; b < batches; b++) {
+        uint32_t* o = out + (size_t)b * B * 8;
+        pair_batch<F, B>(xc, yc, mem, 1, 0, [&](int i, const typename F::fe& x) { F::to_words(x, o + (size_t)i * 8); });
+    }
+}
+
+template <class F>
+static bool host_pair_walk_b(int batch, const uint32_t* upc, const uint32_t base[8], uint32_t batches,
+                             uint32_t* out) {
+    switch (batch) {
+        case 4: host_pair_walk<F, 4>(upc, base, batches, out); return true;
+        case 8: host_pair_walk<F, 8>(upc, base, batches, out); return true;
+        case 16: host_pair_walk<F, 16>(upc, base, batches, out); return true;
+        case 32: host_pair_walk<F, 32>(upc, base, batches, out); return true;
+        case 64: host_pair_walk<F, 64>(upc, base, batches, out); return true;
+        case 128: host_pair_walk<F, 128>(upc, base, batches, out); return true;
+        case 256: host_pair_walk<F, 256>(upc, base, batches, out); return true;
+        default: return false;
+    }
+}
+
+static bool host_pair_walk_run(int layout, int batch, const uint32_t* upc, const uint32_t base[8], uint32_t batches,
+                        uint32_t* out) {
+    if (layout == LXV_LAYOUT_8X32) return host_pair_walk_b<Field8>(batch, upc, base, batches, out);
+    if (layout == LXV_LAYOUT_10X26) return host_pair_walk_b<Field10>(batch, upc, base, batches, out);
+    return false;
+}
+
+}  // namespace lxv
+
+extern "C" int lxv_debug_pair_walk(int layout, uint32_t batch, const uint8_t start[32], const uint32_t* upc,
+                                   uint32_t batches, uint32_t* out) {
+    if (!out || (!upc && (!start || !start_scalar_ok(start)))) return LXV_E_ARG;
+    if (batch < 4 || batch > 256 || (batch & (batch - 1))) return LXV_E_ARG;
+    uint32_t base[8] = {0};
+    if (!upc) scalar_div8_plus(start, (int64_t)batch / 2 - 1, base);
+    return host_pair_walk_run(layout, (int)batch, upc, base, batches, out) ? LXV_OK : LXV_E_ARG;
 }

@@ -42,7 +42,7 @@ struct WalkParams {
 };
 
 template <class F>
-LXV_D void store_fe(uint2* base, uint32_t lanes, uint32_t lane, const typename F::fe& a) {
+LXV_HD void store_fe(uint2* base, uint32_t lanes, uint32_t lane, const typename F::fe& a) {
     uint32_t w[F::kLimbs];
     F::store(a, w);
 #pragma unroll
@@ -50,7 +50,7 @@ LXV_D void store_fe(uint2* base, uint32_t lanes, uint32_t lane, const typename F
 }
 
 template <class F>
-LXV_D typename F::fe load_fe(const uint2* base, uint32_t lanes, uint32_t lane) {
+LXV_HD typename F::fe load_fe(const uint2* base, uint32_t lanes, uint32_t lane) {
     uint32_t w[F::kLimbs];
 #pragma unroll
     for (int i = 0; i < F::kLimbs / 2; i++) {
@@ -62,7 +62,7 @@ LXV_D typename F::fe load_fe(const uint2* base, uint32_t lanes, uint32_t lane) {
 }
 
 template <class F>
-LXV_D void store_state(uint32_t* st, uint32_t lanes, uint32_t lane, int which, const typename F::fe& a) {
+LXV_HD void store_state(uint32_t* st, uint32_t lanes, uint32_t lane, int which, const typename F::fe& a) {
     uint32_t w[F::kLimbs];
     F::store(a, w);
 #pragma unroll
@@ -70,7 +70,7 @@ LXV_D void store_state(uint32_t* st, uint32_t lanes, uint32_t lane, int which, c
 }
 
 template <class F>
-LXV_D typename F::fe load_state(const uint32_t* st, uint32_t lanes, uint32_t lane, int which) {
+LXV_HD typename F::fe load_state(const uint32_t* st, uint32_t lanes, uint32_t lane, int which) {
     uint32_t w[F::kLimbs];
 #pragma unroll
     for (int i = 0; i < F::kLimbs; i++) w[i] = st[(size_t)(which * F::kLimbs + i) * lanes + lane];
@@ -222,7 +222,7 @@ LXV_D void init_lane_from_u(uint32_t* state, uint64_t* ns, uint32_t* xor_acc, co
 // The X25519 ladder over Q = u(qw) for the scalar m = base + off (base as 8
 // little-endian words, off < 2^63, no wrap): (X:Z) of mQ and (m + 1)Q.
 template <class F>
-LXV_D void ladder_mq(const uint32_t base[8], uint64_t off, const uint32_t qw[8], typename F::fe& x2,
+LXV_HD void ladder_mq(const uint32_t base[8], uint64_t off, const uint32_t qw[8], typename F::fe& x2,
                      typename F::fe& z2, typename F::fe& x3, typename F::fe& z3) {
     typedef typename F::fe fe;
     uint32_t m[8];

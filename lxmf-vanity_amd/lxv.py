@@ -29,6 +29,8 @@ LXV_OK = 0
 LXV_E_BUSY = -5
 LAYOUT_8X32 = 0
 LAYOUT_10X26 = 1
+WALK_PAIRS = 0                  # batch keys around an affine centre (the default)
+WALK_PROJECTIVE = 1             # the reference's chained x-only differential additions
 MAX_HITS = 4096                 # hits recorded per launch (src/gpu/mod.rs:23)
 TARGET_LAUNCH_S = 0.5           # src/gpu/mod.rs:26
 
@@ -40,7 +42,7 @@ class GpuUnavailable(RuntimeError):
 class OpenOpts(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("device", ctypes.c_int32), ("threads", ctypes.c_uint32),
                 ("batch", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32), ("layout", ctypes.c_uint32),
-                ("subwalks", ctypes.c_uint32)]
+                ("subwalks", ctypes.c_uint32), ("walk", ctypes.c_uint32)]
 
 
 class LaunchArgs(ctypes.Structure):
@@ -62,7 +64,8 @@ class LaunchStats(ctypes.Structure):
 class MultiOpts(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("device", ctypes.c_int32), ("max_searches", ctypes.c_uint32),
                 ("batch", ctypes.c_uint32), ("layout", ctypes.c_uint32), ("no_graph", ctypes.c_uint32),
-                ("pool_bytes", ctypes.c_uint64), ("max_vlanes", ctypes.c_uint32), ("lane_keys_log2", ctypes.c_uint32)]
+                ("pool_bytes", ctypes.c_uint64), ("max_vlanes", ctypes.c_uint32), ("lane_keys_log2", ctypes.c_uint32),
+                ("walk", ctypes.c_uint32)]
 
 
 class SearchSpec(ctypes.Structure):
@@ -138,6 +141,7 @@ def lib():
     L.lxv_multi_last_error.restype = ctypes.c_char_p
     L.lxv_debug_fe.argtypes = [ctypes.c_int, ctypes.c_int, u32p, u32p, u32p, ctypes.c_uint32, ctypes.c_int]
     L.lxv_debug_address.argtypes = [u32p, u32p, u32p, u32p, ctypes.c_uint32, ctypes.c_int]
+    L.lxv_debug_pair_walk.argtypes = [ctypes.c_int, ctypes.c_uint32, u8p, u32p, ctypes.c_uint32, u32p]
     _lib = L
     return L
 
@@ -219,12 +223,13 @@ def words_to_bytes_le(words):
 class Backend:
     """One GPU context: Backend::{describe, threads, batch, init, launch}."""
 
-    def __init__(self, threads=0, batch=64, waves_per_cu=0, layout=LAYOUT_8X32, device=0, subwalks=0):
+    def __init__(self, threads=0, batch=64, waves_per_cu=0, layout=LAYOUT_8X32, device=0, subwalks=0,
+                 walk=WALK_PAIRS):
         """subwalks=K: K affine sub-walks per lane (threads then counts lanes
         and self.threads is lanes x K; batch is 1 key per thread per launch
-        iteration)."""
+        iteration). walk: WALK_PAIRS or WALK_PROJECTIVE for one walk per lane."""
         o = OpenOpts(ctypes.sizeof(OpenOpts), device, threads, 0 if subwalks else batch, waves_per_cu, layout,
-                     subwalks)
+                     subwalks, walk)
         h = ctypes.c_void_p()
         rc = lib().lxv_open(ctypes.byref(o), ctypes.byref(h))
         if rc != LXV_OK:
@@ -382,9 +387,9 @@ class Multi:
     admit_s, done_s (seconds since open)."""
 
     def __init__(self, max_searches=0, batch=256, layout=LAYOUT_8X32, no_graph=False, pool_bytes=0,
-                 max_vlanes=0, lane_keys_log2=0, device=0):
+                 max_vlanes=0, lane_keys_log2=0, device=0, walk=WALK_PAIRS):
         o = MultiOpts(ctypes.sizeof(MultiOpts), device, max_searches, batch, layout, 1 if no_graph else 0,
-                      pool_bytes, max_vlanes, lane_keys_log2)
+                      pool_bytes, max_vlanes, lane_keys_log2, walk)
         h = ctypes.c_void_p()
         if lib().lxv_multi_open(ctypes.byref(o), ctypes.byref(h)) != LXV_OK:
             raise GpuUnavailable(lib().lxv_multi_last_error(None).decode())
@@ -461,3 +466,15 @@ def debug_address(ed_words, nh_words, u_words, on_gpu):
     if rc != LXV_OK:
         raise RuntimeError("lxv_debug_address failed (%d)" % rc)
     return list(O[:4 * n])
+
+
+def debug_pair_walk(batch, batches, start=None, u_prev_cur=None, layout=LAYOUT_8X32):
+    """The pair walk of one lane on the CPU (the host build of the device
+    code): u(k + 8n) for n < batches x batch as 32-byte encodings, from a
+    start scalar (the lxv_init_seeded set-up) or from 16 words u(k-8), u(k)."""
+    out = (ctypes.c_uint32 * (batches * batch * 8))()
+    upc = (ctypes.c_uint32 * 16)(*u_prev_cur) if u_prev_cur is not None else None
+    rc = lib().lxv_debug_pair_walk(layout, batch, start, upc, batches, out)
+    if rc != LXV_OK:
+        raise RuntimeError("lxv_debug_pair_walk failed (%d)" % rc)
+    return [words_to_bytes_le(out[8 * i:8 * i + 8]) for i in range(batches * batch)]

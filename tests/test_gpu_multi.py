@@ -45,9 +45,10 @@ def oracle_first_hit(start, ed_pub, prefix, vlanes, upto_block):
     return best[3], best[4]
 
 
-def test_answers_are_first_hits_in_walk_order():
+@pytest.mark.parametrize("walk", [lxv.WALK_PAIRS, lxv.WALK_PROJECTIVE], ids=["pairs", "projective"])
+def test_answers_are_first_hits_in_walk_order(walk):
     searches = workload(range(100, 112), [2, 3] * 6)
-    res, st = run(searches)
+    res, st = run(searches, walk=walk)
     for i, (start, ed_pub, prefix) in enumerate(searches):
         r = res[i]
         assert r["status"] == 1
@@ -101,9 +102,12 @@ def test_graph_replay_matches_eager_launches():
         assert (g_res[i]["g"], g_res[i]["address"]) == (e_res[i]["g"], e_res[i]["address"])
 
 
-def test_admission_is_bounded_by_device_memory():
-    # room for three 64-lane searches (8 x 32-bit limbs: 128 bytes per lane)
-    m = lxv.Multi(pool_bytes=3 * 64 * 128)
+@pytest.mark.parametrize("walk,lane_bytes", [(lxv.WALK_PAIRS, 64), (lxv.WALK_PROJECTIVE, 128)],
+                         ids=["pairs", "projective"])
+def test_admission_is_bounded_by_device_memory(walk, lane_bytes):
+    # room for three 64-lane searches (8 x 32-bit limbs: 2 or 4 field
+    # elements of state per lane)
+    m = lxv.Multi(pool_bytes=3 * 64 * lane_bytes, walk=walk)
     try:
         searches = workload(range(500, 510), [3] * 10)
         for i, (st, ed, pf) in enumerate(searches):

@@ -68,11 +68,15 @@ def test_gpu_address_hash_matches_oracle():
         assert lxv.words_to_bytes_be(out[4 * i:4 * i + 4]) == orc.lxmf_address(u, ed_pub)
 
 
-# The reference's form (one projective walk per lane, 64 keys per inversion)
-# in both limb layouts, and the sub-walk form (64 affine walks per lane: a
-# thread is one of them, one key per thread per iteration).
+# One walk per lane, 64 keys per inversion, in both limb layouts and both
+# walk forms (the pair walk, the default, and the reference's projective
+# walk), and the sub-walk form (64 affine walks per lane: a thread is one of
+# them, one key per thread per iteration).
 FORMS = {"10x26": dict(threads=4096, batch=64, layout=lxv.LAYOUT_10X26),
          "8x32": dict(threads=4096, batch=64, layout=lxv.LAYOUT_8X32),
+         "10x26-proj": dict(threads=4096, batch=64, layout=lxv.LAYOUT_10X26, walk=lxv.WALK_PROJECTIVE),
+         "8x32-proj": dict(threads=4096, batch=64, layout=lxv.LAYOUT_8X32, walk=lxv.WALK_PROJECTIVE),
+         "8x32-b16": dict(threads=4096, batch=16, layout=lxv.LAYOUT_8X32),
          "8x32-sw64": dict(threads=512, subwalks=64, layout=lxv.LAYOUT_8X32)}
 
 
@@ -94,7 +98,10 @@ def test_walk_dump_matches_x25519(small_backend):
     rng = random.Random(42)
     lanes = [0, 1, 63, 64, 255] + rng.sample(range(256), 20)
     for t in lanes:
-        for i in [i for i in list(range(8)) + [31, 32, 62, 63] if i < B]:
+        # every key of the batch on five lanes (every position of a pair
+        # walk's batch), a sample on the others
+        keys = range(B) if t in (0, 1, 63, 64, 255) else [i for i in list(range(8)) + [31, 32, 62, 63] if i < B]
+        for i in keys:
             k = orc.step_scalar(start, t * STRIDE + i)
             got = lxv.words_to_bytes_le(dump[(t * B + i) * 8:(t * B + i) * 8 + 8])
             assert got == orc.x25519(k), (t, i)
@@ -135,7 +142,9 @@ def test_walk_continues_across_launches(small_backend):
     assert b.checksum() == x_two
 
 
-@pytest.mark.parametrize("form", [dict(threads=256, batch=64), dict(threads=64, subwalks=64)], ids=["lanes", "sw64"])
+@pytest.mark.parametrize("form", [dict(threads=256, batch=64), dict(threads=256, batch=64, walk=lxv.WALK_PROJECTIVE),
+                                  dict(threads=256, batch=256, layout=lxv.LAYOUT_10X26),
+                                  dict(threads=64, subwalks=64)], ids=["pairs", "proj", "pairs-10x26-b256", "sw64"])
 def test_reference_style_init_from_host_start_points(form):
     # Backend::init with u(k-8), u(k) per thread from independent random
     # starts (src/gpu/mod.rs:210-228), computed with the product's host X25519.
@@ -213,7 +222,10 @@ def test_stop_ends_a_long_launch_and_the_walk_resumes(small_backend):
 
 @pytest.mark.parametrize("form", [dict(threads=65536, batch=256, layout=lxv.LAYOUT_8X32),
                                   dict(threads=65536, batch=256, layout=lxv.LAYOUT_10X26),
-                                  dict(threads=1024, subwalks=64)], ids=["8x32", "10x26", "8x32-sw64"])
+                                  dict(threads=65536, batch=256, layout=lxv.LAYOUT_8X32, walk=lxv.WALK_PROJECTIVE),
+                                  dict(threads=65536, batch=256, layout=lxv.LAYOUT_10X26, walk=lxv.WALK_PROJECTIVE),
+                                  dict(threads=1024, subwalks=64)],
+                         ids=["8x32", "10x26", "8x32-proj", "10x26-proj", "8x32-sw64"])
 def test_config0_workload_matches_golden(form):
     # The small workload at full size: 2^24 consecutive keys of workload
     # seed 0 split evenly over the threads (thread t walks g in [m t, m t + m)
