@@ -57,7 +57,9 @@ extern "C" {
 
 typedef struct lxv_ctx lxv_ctx;
 
-/* Options for lxv_open. Zero-initialise, set `size`, override what you need. */
+/* Options for lxv_open. Zero-initialise, set `size`, override what you need.
+ * A shorter struct from an older header (ending at `layout`) is accepted: the
+ * fields it lacks take their zero defaults. */
 typedef struct lxv_open_opts {
     uint32_t size;         /* sizeof(lxv_open_opts) */
     int32_t device;        /* HIP device ordinal; negative means 0 */
@@ -233,7 +235,8 @@ void lxv_seeded_search(uint64_t seed, int nibbles, uint8_t start[32], uint8_t ed
 typedef struct lxv_multi lxv_multi;
 
 typedef struct lxv_multi_opts {
-    uint32_t size;          /* sizeof(lxv_multi_opts) */
+    uint32_t size;          /* sizeof(lxv_multi_opts); a struct ending at lane_keys_log2
+                               (an older header) is accepted, `walk` then defaults */
     int32_t device;         /* HIP device ordinal; negative means 0 */
     uint32_t max_searches;  /* slots resident at once; 0 = 1024 */
     uint32_t batch;         /* keys per inversion: 64 or 256; 0 = 256 */

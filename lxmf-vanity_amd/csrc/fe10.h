@@ -10,6 +10,7 @@
 // valid input to fe10_mul/fe10_sq (column sums stay below 2^61).
 #pragma once
 
+#include "fe_divsteps.h"
 #include "lxv_common.h"
 
 namespace lxv {
@@ -191,7 +192,7 @@ namespace lxv {
 
 // z^(p - 2) = z^-1 by the addition chain of ref10 (254 squarings, 11
 // multiplications), as the reference's fe_invert (src/gpu/common.h:115-128).
-LXV_HD Fe10 fe10_invert(const Fe10& z) {
+LXV_HD Fe10 fe10_invert_chain(const Fe10& z) {
     const Fe10 z2 = fe10_sq(z);
     const Fe10 z9 = fe10_mul(fe10_sq_n(z2, 2), z);
     const Fe10 z11 = fe10_mul(z9, z2);
@@ -224,7 +225,12 @@ struct Field10 {
     }
     LXV_HD static fe mul(const fe& a, const fe& b) { return fe10_mul(a, b); }
     LXV_HD static fe sq(const fe& a) { return fe10_sq(a); }
-    LXV_HD static fe invert(const fe& a) { return fe10_invert(a); }
+    LXV_HD static fe invert(const fe& a) {
+        uint32_t w[8], r[8];
+        fe10_to_words(a, w);
+        divsteps_invert(w, r);
+        return fe10_from_words(r);
+    }
     LXV_HD static fe from_words(const uint32_t w[8]) { return fe10_from_words(w); }
     LXV_HD static void to_words(const fe& a, uint32_t out[8]) { fe10_to_words(a, out); }
     // a24 * a for the ladder (a24 = 121665 < 2^17): a plain limb scale and a carry.

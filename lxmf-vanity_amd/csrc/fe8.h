@@ -6,6 +6,7 @@
 // borrow back in at once, so every value stays below 2^256.
 #pragma once
 
+#include "fe_divsteps.h"
 #include "lxv_common.h"
 
 namespace lxv {
@@ -67,17 +68,90 @@ LXV_HD void fe8_fold(uint32_t t[8], uint32_t c) {
     t[0] += carry * 38u;
 }
 
+// Sums and differences: one carry chain plus the fold of the carry (borrow)
+// as 38, propagated by a second chain. On the device that is 19 VALU
+// instructions, the chains carrying through VCC (written out because the
+// compiler materialises the second chain's carries one by one); v_cndmask
+// reads VCC two wait states after the VALU write, as gfx950 requires.
 LXV_HD Fe8 fe8_add(const Fe8& a, const Fe8& b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    Fe8 r;
+    uint32_t t;
+    asm("v_add_co_u32_e32 %[r0], vcc, %[a0], %[b0]\n"
+        "v_addc_co_u32_e32 %[r1], vcc, %[a1], %[b1], vcc\n"
+        "v_addc_co_u32_e32 %[r2], vcc, %[a2], %[b2], vcc\n"
+        "v_addc_co_u32_e32 %[r3], vcc, %[a3], %[b3], vcc\n"
+        "v_addc_co_u32_e32 %[r4], vcc, %[a4], %[b4], vcc\n"
+        "v_addc_co_u32_e32 %[r5], vcc, %[a5], %[b5], vcc\n"
+        "v_addc_co_u32_e32 %[r6], vcc, %[a6], %[b6], vcc\n"
+        "v_addc_co_u32_e32 %[r7], vcc, %[a7], %[b7], vcc\n"
+        "s_nop 1\n"
+        "v_cndmask_b32_e32 %[t], 0, %[k38], vcc\n"
+        "v_add_co_u32_e32 %[r0], vcc, %[r0], %[t]\n"
+        "v_addc_co_u32_e32 %[r1], vcc, 0, %[r1], vcc\n"
+        "v_addc_co_u32_e32 %[r2], vcc, 0, %[r2], vcc\n"
+        "v_addc_co_u32_e32 %[r3], vcc, 0, %[r3], vcc\n"
+        "v_addc_co_u32_e32 %[r4], vcc, 0, %[r4], vcc\n"
+        "v_addc_co_u32_e32 %[r5], vcc, 0, %[r5], vcc\n"
+        "v_addc_co_u32_e32 %[r6], vcc, 0, %[r6], vcc\n"
+        "v_addc_co_u32_e32 %[r7], vcc, 0, %[r7], vcc\n"
+        "s_nop 1\n"
+        "v_cndmask_b32_e32 %[t], 0, %[k38], vcc\n"
+        "v_add_u32_e32 %[r0], %[r0], %[t]\n"
+        : [r0] "=&v"(r.v[0]), [r1] "=&v"(r.v[1]), [r2] "=&v"(r.v[2]), [r3] "=&v"(r.v[3]), [r4] "=&v"(r.v[4]),
+          [r5] "=&v"(r.v[5]), [r6] "=&v"(r.v[6]), [r7] "=&v"(r.v[7]), [t] "=&v"(t)
+        : [a0] "v"(a.v[0]), [a1] "v"(a.v[1]), [a2] "v"(a.v[2]), [a3] "v"(a.v[3]), [a4] "v"(a.v[4]),
+          [a5] "v"(a.v[5]), [a6] "v"(a.v[6]), [a7] "v"(a.v[7]), [b0] "v"(b.v[0]), [b1] "v"(b.v[1]),
+          [b2] "v"(b.v[2]), [b3] "v"(b.v[3]), [b4] "v"(b.v[4]), [b5] "v"(b.v[5]), [b6] "v"(b.v[6]),
+          [b7] "v"(b.v[7]), [k38] "v"(38u)
+        : "vcc");
+    return r;
+#else
     Fe8 r;
     uint32_t c = 0;
 #pragma unroll
     for (int i = 0; i < 8; i++) r.v[i] = addc32(a.v[i], b.v[i], c, &c);
     fe8_fold(r.v, c);
     return r;
+#endif
 }
 
-// a - b: a borrow means 2^256 was added, i.e. 38 too many (mod p).
+// a - b: a borrow means 2^256 was added, i.e. 38 too many (mod p); a second
+// borrow leaves the value >= 2^256 - 38, so one more 38 cannot borrow.
 LXV_HD Fe8 fe8_sub(const Fe8& a, const Fe8& b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    Fe8 r;
+    uint32_t t;
+    asm("v_sub_co_u32_e32 %[r0], vcc, %[a0], %[b0]\n"
+        "v_subb_co_u32_e32 %[r1], vcc, %[a1], %[b1], vcc\n"
+        "v_subb_co_u32_e32 %[r2], vcc, %[a2], %[b2], vcc\n"
+        "v_subb_co_u32_e32 %[r3], vcc, %[a3], %[b3], vcc\n"
+        "v_subb_co_u32_e32 %[r4], vcc, %[a4], %[b4], vcc\n"
+        "v_subb_co_u32_e32 %[r5], vcc, %[a5], %[b5], vcc\n"
+        "v_subb_co_u32_e32 %[r6], vcc, %[a6], %[b6], vcc\n"
+        "v_subb_co_u32_e32 %[r7], vcc, %[a7], %[b7], vcc\n"
+        "s_nop 1\n"
+        "v_cndmask_b32_e32 %[t], 0, %[k38], vcc\n"
+        "v_sub_co_u32_e32 %[r0], vcc, %[r0], %[t]\n"
+        "v_subbrev_co_u32_e32 %[r1], vcc, 0, %[r1], vcc\n"
+        "v_subbrev_co_u32_e32 %[r2], vcc, 0, %[r2], vcc\n"
+        "v_subbrev_co_u32_e32 %[r3], vcc, 0, %[r3], vcc\n"
+        "v_subbrev_co_u32_e32 %[r4], vcc, 0, %[r4], vcc\n"
+        "v_subbrev_co_u32_e32 %[r5], vcc, 0, %[r5], vcc\n"
+        "v_subbrev_co_u32_e32 %[r6], vcc, 0, %[r6], vcc\n"
+        "v_subbrev_co_u32_e32 %[r7], vcc, 0, %[r7], vcc\n"
+        "s_nop 1\n"
+        "v_cndmask_b32_e32 %[t], 0, %[k38], vcc\n"
+        "v_sub_u32_e32 %[r0], %[r0], %[t]\n"
+        : [r0] "=&v"(r.v[0]), [r1] "=&v"(r.v[1]), [r2] "=&v"(r.v[2]), [r3] "=&v"(r.v[3]), [r4] "=&v"(r.v[4]),
+          [r5] "=&v"(r.v[5]), [r6] "=&v"(r.v[6]), [r7] "=&v"(r.v[7]), [t] "=&v"(t)
+        : [a0] "v"(a.v[0]), [a1] "v"(a.v[1]), [a2] "v"(a.v[2]), [a3] "v"(a.v[3]), [a4] "v"(a.v[4]),
+          [a5] "v"(a.v[5]), [a6] "v"(a.v[6]), [a7] "v"(a.v[7]), [b0] "v"(b.v[0]), [b1] "v"(b.v[1]),
+          [b2] "v"(b.v[2]), [b3] "v"(b.v[3]), [b4] "v"(b.v[4]), [b5] "v"(b.v[5]), [b6] "v"(b.v[6]),
+          [b7] "v"(b.v[7]), [k38] "v"(38u)
+        : "vcc");
+    return r;
+#else
     Fe8 r;
     uint32_t bw = 0;
 #pragma unroll
@@ -89,6 +163,7 @@ LXV_HD Fe8 fe8_sub(const Fe8& a, const Fe8& b) {
     // a second wrap leaves r >= 2^256 - 38: one more 38 cannot borrow
     r.v[0] -= bw2 * 38u;
     return r;
+#endif
 }
 
 // 512-bit product to a value below 2^256 (host build of the multiply).
@@ -196,160 +271,12 @@ LXV_HD Fe8 fe8_invert_chain(const Fe8& z) {
     return fe8_mul(fe8_sq_n(z2_250_0, 5), z11);
 }
 
-// ---------------------------------------------------------------------------
-// Inversion by divsteps (Bernstein and Yang, "Fast constant-time gcd
-// computation and modular inversion", 2019), in the half-delta form: 20
-// batches of 30 steps, 600 in all, above the 590 that bound every input below
-// 2^256 for this form. The steps are the same instructions for every lane
-// (no data-dependent branch), and a batch costs about 21 VALU operations per
-// step on 32-bit words plus two 2x2 matrix products on signed 30-bit limbs:
-// about a third of the 254 squarings of the addition chain.
-//
-// A batch works on the low 32 bits of f and g only (30 steps consume 30 low
-// bits, and each step reads only bit 0 of g), accumulating the transition
-// matrix (u v; q r) with 2^30 (f', g') = (u f + v g, q f + r g). The matrix
-// is then applied to the full f, g (exact division by 2^30) and to the
-// coefficients d, e with f = d z, g = e z (mod p), where the division by 2^30
-// is made exact by adding a multiple of p chosen from p^-1 mod 2^30; adding
-// u p (v p) when d (e) is negative keeps d, e in (-2p, p). At the end g = 0,
-// f = +-1 and 1/z = d f.
-
-struct S30 {
-    int32_t v[9];  // v[0..7] in [0, 2^30) after an update, v[8] signed
-};
-
-constexpr uint32_t kM30 = (1u << 30) - 1u;
-constexpr uint32_t kPInv30 = 0x179435e5u;  // p^-1 mod 2^30 (p = 2^255 - 19)
-
-LXV_HD int32_t s30_p_limb(int i) { return i == 0 ? (int32_t)((1u << 30) - 19u) : (i == 8 ? 32767 : (int32_t)kM30); }
-
-// 30 divsteps on the low words of f and g; returns zeta = -(delta + 1/2)
-// after them and the matrix t = (u, v, q, r).
-LXV_HD uint32_t fe8_divsteps30(uint32_t zeta, uint32_t f, uint32_t g, int32_t t[4]) {
-    uint32_t u = 1, v = 0, q = 0, r = 1;
-#pragma unroll
-    for (int i = 0; i < 30; i++) {
-        const uint32_t c1 = (uint32_t)((int32_t)zeta >> 31);  // delta > 0
-        const uint32_t c2 = 0u - (g & 1u);                    // g odd
-        const uint32_t n1 = c1 & 1u;
-        // with c1: (f, g) -> (g, -f) first; with c2: g += f; then g /= 2
-        const uint32_t x = (f ^ c1) + n1, y = (u ^ c1) + n1, z = (v ^ c1) + n1;
-        g += x & c2;
-        q += y & c2;
-        r += z & c2;
-        const uint32_t c3 = c1 & c2;
-        zeta = (zeta ^ c3) - 1u;
-        f += g & c3;
-        u = (u + (q & c3)) << 1;
-        v = (v + (r & c3)) << 1;
-        g >>= 1;
-    }
-    t[0] = (int32_t)u;
-    t[1] = (int32_t)v;
-    t[2] = (int32_t)q;
-    t[3] = (int32_t)r;
-    return zeta;
-}
-
-// (f, g) <- (u f + v g, q f + r g) / 2^30.
-LXV_HD void s30_update_fg(S30& f, S30& g, const int32_t t[4]) {
-    int64_t cf = (int64_t)t[0] * f.v[0] + (int64_t)t[1] * g.v[0];
-    int64_t cg = (int64_t)t[2] * f.v[0] + (int64_t)t[3] * g.v[0];
-    cf >>= 30;
-    cg >>= 30;
-#pragma unroll
-    for (int i = 1; i < 9; i++) {
-        cf += (int64_t)t[0] * f.v[i] + (int64_t)t[1] * g.v[i];
-        cg += (int64_t)t[2] * f.v[i] + (int64_t)t[3] * g.v[i];
-        f.v[i - 1] = (int32_t)((uint32_t)cf & kM30);
-        g.v[i - 1] = (int32_t)((uint32_t)cg & kM30);
-        cf >>= 30;
-        cg >>= 30;
-    }
-    f.v[8] = (int32_t)cf;
-    g.v[8] = (int32_t)cg;
-}
-
-// (d, e) <- (u d + v e, q d + r e) / 2^30 (mod p), kept in (-2p, p).
-LXV_HD void s30_update_de(S30& d, S30& e, const int32_t t[4]) {
-    const int32_t u = t[0], v = t[1], q = t[2], r = t[3];
-    const int32_t sd = d.v[8] >> 31, se = e.v[8] >> 31;
-    int32_t md = (u & sd) + (v & se);
-    int32_t me = (q & sd) + (r & se);
-    int64_t cd = (int64_t)u * d.v[0] + (int64_t)v * e.v[0];
-    int64_t ce = (int64_t)q * d.v[0] + (int64_t)r * e.v[0];
-    md -= (int32_t)((kPInv30 * (uint32_t)cd + (uint32_t)md) & kM30);
-    me -= (int32_t)((kPInv30 * (uint32_t)ce + (uint32_t)me) & kM30);
-    cd += (int64_t)s30_p_limb(0) * md;
-    ce += (int64_t)s30_p_limb(0) * me;
-    cd >>= 30;
-    ce >>= 30;
-#pragma unroll
-    for (int i = 1; i < 9; i++) {
-        cd += (int64_t)u * d.v[i] + (int64_t)v * e.v[i] + (int64_t)s30_p_limb(i) * md;
-        ce += (int64_t)q * d.v[i] + (int64_t)r * e.v[i] + (int64_t)s30_p_limb(i) * me;
-        d.v[i - 1] = (int32_t)((uint32_t)cd & kM30);
-        e.v[i - 1] = (int32_t)((uint32_t)ce & kM30);
-        cd >>= 30;
-        ce >>= 30;
-    }
-    d.v[8] = (int32_t)cd;
-    e.v[8] = (int32_t)ce;
-}
-
-// Carries every limb into [0, 2^30) except the signed top one.
-LXV_HD void s30_normalize(S30& a) {
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        a.v[i + 1] += a.v[i] >> 30;
-        a.v[i] = (int32_t)((uint32_t)a.v[i] & kM30);
-    }
-}
-
-// a + p if a < 0 (a normalized), normalized again.
-LXV_HD void s30_add_p_if_negative(S30& a) {
-    const int32_t m = a.v[8] >> 31;
-#pragma unroll
-    for (int i = 0; i < 9; i++) a.v[i] += s30_p_limb(i) & m;
-    s30_normalize(a);
-}
-
+// 1/z by divsteps (fe_divsteps.h).
 LXV_HD Fe8 fe8_invert(const Fe8& z) {
     uint32_t w[8];
-    fe8_to_words(z, w);  // canonical, so g < p
-    S30 f, g, d, e;
-#pragma unroll
-    for (int i = 0; i < 9; i++) {
-        const int k = (30 * i) / 32, s = (30 * i) % 32;
-        const uint32_t lo = w[k], hi = k + 1 < 8 ? w[k + 1] : 0u;
-        g.v[i] = (int32_t)(funnel32(hi, lo, (uint32_t)s) & kM30);
-        f.v[i] = s30_p_limb(i);
-        d.v[i] = 0;
-        e.v[i] = i == 0 ? 1 : 0;
-    }
-    uint32_t zeta = 0xffffffffu;  // delta = 1/2
-    for (int b = 0; b < 20; b++) {
-        int32_t t[4];
-        zeta = fe8_divsteps30(zeta, (uint32_t)f.v[0] | ((uint32_t)f.v[1] << 30),
-                              (uint32_t)g.v[0] | ((uint32_t)g.v[1] << 30), t);
-        s30_update_de(d, e, t);
-        s30_update_fg(f, g, t);
-    }
-    // 1/z = d f with f = +-1: negate d when f < 0, then bring it into [0, 2p)
-    const int32_t neg = f.v[8] >> 31;
-#pragma unroll
-    for (int i = 0; i < 9; i++) d.v[i] = (d.v[i] ^ neg) - neg;
-    s30_normalize(d);
-    s30_add_p_if_negative(d);
-    s30_add_p_if_negative(d);
+    fe8_to_words(z, w);
     Fe8 r;
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const int i = (32 * j) / 30, s = (32 * j) % 30;
-        const uint32_t lo = (uint32_t)d.v[i] >> s;
-        const uint32_t hi = i + 1 < 9 ? (uint32_t)d.v[i + 1] << (30 - s) : 0u;
-        r.v[j] = lo | hi;
-    }
+    divsteps_invert(w, r.v);
     return r;
 }
 

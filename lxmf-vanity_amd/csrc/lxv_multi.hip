@@ -3,6 +3,8 @@
 // hipGraph-captured relaunch of the persistent kernel, and answer selection.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -215,8 +217,11 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
     *out = nullptr;
     lxv_multi_opts o{};
     if (opts) {
-        if (opts->size < sizeof(lxv_multi_opts)) return mfail(nullptr, LXV_E_ARG, "lxv_multi_open: opts->size too small");
-        o = *opts;
+        // A shorter struct (an older header) leaves the fields it lacks
+        // (those after `lane_keys_log2`) at their zero defaults.
+        if (opts->size < offsetof(lxv_multi_opts, walk))
+            return mfail(nullptr, LXV_E_ARG, "lxv_multi_open: opts->size too small");
+        std::memcpy(&o, opts, opts->size < sizeof o ? opts->size : sizeof o);
     }
     auto* m = new (std::nothrow) lxv_multi();
     if (!m) return mfail(nullptr, LXV_E_NOMEM, "lxv_multi_open: out of host memory");

@@ -3,6 +3,7 @@
 // of the search driver. Host code only; the kernels are in lxv_kernels.hip.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -98,8 +99,11 @@ extern "C" int lxv_open(const lxv_open_opts* opts, lxv_ctx** out) {
     *out = nullptr;
     lxv_open_opts o{};
     if (opts) {
-        if (opts->size < sizeof(lxv_open_opts)) return fail(nullptr, LXV_E_ARG, "lxv_open: opts->size too small");
-        o = *opts;
+        // Callers built against an older header pass a shorter struct: the
+        // fields it lacks (those after `layout`) keep their zero defaults.
+        if (opts->size < offsetof(lxv_open_opts, subwalks))
+            return fail(nullptr, LXV_E_ARG, "lxv_open: opts->size too small");
+        std::memcpy(&o, opts, opts->size < sizeof o ? opts->size : sizeof o);
     }
     auto* c = new (std::nothrow) lxv_ctx();
     if (!c) return fail(nullptr, LXV_E_NOMEM, "lxv_open: out of host memory");
