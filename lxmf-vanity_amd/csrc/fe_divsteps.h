@@ -165,6 +165,19 @@ LXV_HD void divsteps_invert(const uint32_t w[8], uint32_t out[8]) {
                           (uint32_t)g.v[0] | ((uint32_t)g.v[1] << 30), t);
         s30_update_de(d, e, t);
         s30_update_fg(f, g, t);
+        // Once g = 0 further steps leave f and d unchanged (mod p): stop when
+        // every lane of the wave is there (random inputs need about 515
+        // steps, so 18 batches; the 20 cover the proven worst case).
+        if (b >= 16) {
+            int32_t any = 0;
+#pragma unroll
+            for (int i = 0; i < 9; i++) any |= g.v[i];
+#if defined(__HIP_DEVICE_COMPILE__)
+            if (__all(any == 0)) break;
+#else
+            if (any == 0) break;
+#endif
+        }
     }
     // 1/w = d f with f = +-1: negate d when f < 0, bring it into [0, 2p),
     // then subtract p once if that leaves it non-negative

@@ -85,7 +85,7 @@ LXV_HD void pair_batch(typename F::fe& xc, typename F::fe& yc, uint2* mem, uint3
     // backward, one key per step: j = 0 the single key (e = H + 1), then
     // e = H .. 1 with C + T_e on odd j and C - T_e on even j, then the
     // centre's own key. The element's inverse is formed on the first key.
-    fe ie, s, ve;
+    fe ie, s;
     int e = H + 1;
     for (int j = 0; j < B; j++) {
         fe x;
@@ -96,14 +96,16 @@ LXV_HD void pair_batch(typename F::fe& xc, typename F::fe& yc, uint2* mem, uint3
         } else if (j == 0 || (j & 1)) {
             e = j == 0 ? H + 1 : H - (j - 1) / 2;
             const fe ue = pair_tab_u<F>(e);
-            ve = pair_tab_v<F>(e);
+            const fe ve = pair_tab_v<F>(e);
             ie = F::mul(inv, load_fe<F>(mem + (size_t)(e - 1) * slot, lanes, lane));
             inv = F::mul(inv, F::sub(ue, xc));
             s = F::tight(F::add(sa, ue));
             x = F::tight(F::sub(F::sq(F::mul(F::sub(ve, yc), ie)), s));
             i = H + e;
         } else {
-            x = F::tight(F::sub(F::sq(F::mul(F::add(ve, yc), ie)), s));
+            // v(T_e) comes again from the table (scalar loads) rather than
+            // across the hash of the first key, which frees 8 VGPRs
+            x = F::tight(F::sub(F::sq(F::mul(F::add(pair_tab_v<F>(e), yc), ie)), s));
             i = H - e;
         }
         emit(i, x);
