@@ -177,6 +177,29 @@ def test_reference_style_init_from_host_start_points(form):
         b.close()
 
 
+@pytest.mark.parametrize("batch", [32, 128, 256])
+@pytest.mark.parametrize("layout", [lxv.LAYOUT_8X32, lxv.LAYOUT_10X26], ids=["8x32", "10x26"])
+def test_pair_walk_every_batch_size(layout, batch):
+    # The pair walk at the batch sizes the small backends above do not use:
+    # every key of the first batch on four lanes, and every hit and the
+    # checksum of two batches on all lanes, against the oracle.
+    b = lxv.Backend(threads=1024, batch=batch, layout=layout)
+    try:
+        start, _, ed_pub, prefix = seeded(60 + batch, 2)
+        b.init_seeded(start, STRIDE)
+        hits, dump = b.launch(lxv.launch_args_for(ed_pub, prefix, iters=2, dump_threads=4, max_hits=1 << 14))
+        for t in range(4):
+            for i in range(batch):
+                k = orc.step_scalar(start, t * STRIDE + i)
+                assert lxv.words_to_bytes_le(dump[(t * batch + i) * 8:(t * batch + i) * 8 + 8]) == orc.x25519(k), (t, i)
+        want, xor = orc.scan_ranges(start, ed_pub, [(t * STRIDE, 2 * batch) for t in range(b.threads)], prefix,
+                                    cap=1 << 14)
+        assert sorted((t * STRIDE + n, a) for t, n, a in hits) == want
+        assert b.checksum() == xor
+    finally:
+        b.close()
+
+
 def test_full_address_prefix_finds_exactly_that_key(small_backend):
     b = small_backend
     start, _, ed_pub, _ = seeded(49, 0)
