@@ -23,6 +23,7 @@ extern "C" {
 #define LXV_FE_INVERT 4
 #define LXV_FE_CANON 5
 #define LXV_FE_MUL121665 6
+#define LXV_FE_INVERT_CHAIN 7  /* z^(p-2) by the reference's addition chain (LXV_FE_INVERT: divsteps) */
 
 /* out[i] = canonical encoding of op(a[i], b[i]) in limb layout `layout`
  * (LXV_LAYOUT_*), for n elements of 8 little-endian words each (bit 255 of

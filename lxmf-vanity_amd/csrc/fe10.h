@@ -231,6 +231,7 @@ struct Field10 {
         divsteps_invert(w, r);
         return fe10_from_words(r);
     }
+    LXV_HD static fe invert_chain(const fe& a) { return fe10_invert_chain(a); }
     LXV_HD static fe from_words(const uint32_t w[8]) { return fe10_from_words(w); }
     LXV_HD static void to_words(const fe& a, uint32_t out[8]) { fe10_to_words(a, out); }
     // a24 * a for the ladder (a24 = 121665 < 2^17): a plain limb scale and a carry.

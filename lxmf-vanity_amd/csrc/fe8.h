@@ -292,6 +292,7 @@ struct Field8 {
     LXV_HD static fe mul(const fe& a, const fe& b) { return fe8_mul(a, b); }
     LXV_HD static fe sq(const fe& a) { return fe8_sq(a); }
     LXV_HD static fe invert(const fe& a) { return fe8_invert(a); }
+    LXV_HD static fe invert_chain(const fe& a) { return fe8_invert_chain(a); }
     LXV_HD static fe from_words(const uint32_t w[8]) { return fe8_from_words(w); }
     LXV_HD static void to_words(const fe& a, uint32_t out[8]) { fe8_to_words(a, out); }
     LXV_HD static fe mul_small(const fe& a, int32_t k) { return fe8_mul_small(a, (uint32_t)k); }

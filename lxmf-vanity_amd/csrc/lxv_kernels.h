@@ -10,7 +10,9 @@ namespace lxv {
 
 struct WalkParams;
 
-enum DebugFeOp { kFeMul = 0, kFeSq = 1, kFeAdd = 2, kFeSub = 3, kFeInvert = 4, kFeCanon = 5, kFeMulSmall = 6 };
+enum DebugFeOp {
+    kFeMul = 0, kFeSq = 1, kFeAdd = 2, kFeSub = 3, kFeInvert = 4, kFeCanon = 5, kFeMulSmall = 6, kFeInvertChain = 7
+};
 
 // out = canonical words of op(a, b); a and b are decoded as 8 little-endian
 // words (bit 255 dropped). Sums and differences are multiplied by one before
@@ -26,6 +28,7 @@ LXV_HD void debug_fe_op(int op, const uint32_t a[8], const uint32_t b[8], uint32
         case kFeAdd: r = F::mul(F::add(x, y), F::one()); break;
         case kFeSub: r = F::mul(F::sub(x, y), F::one()); break;
         case kFeInvert: r = F::invert(x); break;
+        case kFeInvertChain: r = F::invert_chain(x); break;
         case kFeMulSmall: r = F::mul_small(x, 121665); break;
         default: r = F::mul(x, F::one()); break;
     }

@@ -511,7 +511,7 @@ extern "C" void lxv_seeded_search(uint64_t seed, int nibbles, uint8_t start[32],
 
 extern "C" int lxv_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n,
                             int on_gpu) {
-    if (!a || !b || !out || op < 0 || op > 6) return LXV_E_ARG;
+    if (!a || !b || !out || op < 0 || op > 7) return LXV_E_ARG;
     if (layout != LXV_LAYOUT_10X26 && layout != LXV_LAYOUT_8X32) return LXV_E_ARG;
     if (!on_gpu) {
         for (uint32_t i = 0; i < n; i++) {

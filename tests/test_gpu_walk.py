@@ -49,7 +49,7 @@ def test_gpu_field_ops_match_big_integers(layout):
     expect = {0: [a * b % P for a, b in zip(vals, bs)], 1: [a * a % P for a in vals],
               2: [(a + b) % P for a, b in zip(vals, bs)], 3: [(a - b) % P for a, b in zip(vals, bs)],
               5: [a % P for a in vals], 6: [a * 121665 % P for a in vals],
-              4: [pow(a, P - 2, P) for a in vals]}
+              4: [pow(a, P - 2, P) for a in vals], 7: [pow(a, P - 2, P) for a in vals]}
     for op, want in expect.items():
         print("field op", op, flush=True)
         got = lxv.debug_fe(op, A, B, on_gpu=True, layout=layout)

@@ -147,11 +147,12 @@ def test_device_field_code_on_host(layout):
         got = lxv.debug_fe(op, A, B, on_gpu=False, layout=layout)
         got = [from_words(got[8 * i:8 * i + 8]) for i in range(len(vals))]
         assert got == want, op
-    # inversion (8x32: divsteps; 10x25.5: the addition chain) against
-    # Fermat's little theorem, 0 included (it maps to 0 either way)
-    inv = lxv.debug_fe(4, A, B, on_gpu=False, layout=layout)
-    for i, a in enumerate(vals):
-        assert from_words(inv[8 * i:8 * i + 8]) == pow(a, P - 2, P)
+    # inversion by divsteps (op 4) and by the reference's addition chain
+    # (op 7) against Fermat's little theorem, 0 included (it maps to 0)
+    for op in (4, 7):
+        inv = lxv.debug_fe(op, A, B, on_gpu=False, layout=layout)
+        for i, a in enumerate(vals):
+            assert from_words(inv[8 * i:8 * i + 8]) == pow(a, P - 2, P), (op, i)
 
 
 @pytest.mark.parametrize("layout", [lxv.LAYOUT_10X26, lxv.LAYOUT_8X32])
