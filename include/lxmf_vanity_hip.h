@@ -63,7 +63,9 @@ typedef struct lxv_ctx lxv_ctx;
 typedef struct lxv_open_opts {
     uint32_t size;         /* sizeof(lxv_open_opts) */
     int32_t device;        /* HIP device ordinal; negative means 0 */
-    uint32_t threads;      /* walk lanes; 0 = CUs x waves_per_cu x 64 */
+    uint32_t threads;      /* walk lanes; 0 = CUs x waves_per_cu x 64 x 8 (eight rounds of
+                              resident waves, so the dispatcher balances the launch;
+                              sub-walk form: CUs x waves_per_cu x 64) */
     uint32_t batch;        /* keys per batched inversion (B); 0 = 64 */
     uint32_t waves_per_cu; /* resident waves per CU the grid is sized for; 0 = 16 */
     uint32_t layout;       /* LXV_LAYOUT_*; 0 (8x32) is the default */

@@ -1,7 +1,9 @@
 """Autotuner of the walk kernel: keys per inversion x waves per CU x limb layout
 (x walk form: the pair walk and the reference's projective walk).
 
-Each point opens the backend with that shape, sets every lane up on the GPU
+Each point opens the backend with that shape (lanes = CUs x waves per CU x 64,
+one resident round, so the waves per CU are the point's occupancy; the
+library's default adds eight rounds of lanes), sets every lane up on the GPU
 and walks a fixed budget of keys of one seeded search (an 8-hex-digit prefix
 from workload seed 2), timed with HIP events on the kernel's stream; every hit
 is re-derived on the CPU. Per point it records keys/s, the kernel's VGPRs and
@@ -24,6 +26,7 @@ import csv
 import ctypes
 import json
 import os
+import re
 import sys
 import time
 
@@ -51,8 +54,19 @@ def kernel_attrs(layout, batch, wpc, walk):
     return {"vgprs": v.value, "scratch_bytes": sc.value, "waves_per_simd": w.value}
 
 
-def run_point(layout, batch, wpc, walk, budget, start, ed_seed, prefix):
-    s = lxv.GpuSearch(start=start, ed_seed=ed_seed, batch=batch, waves_per_cu=wpc, layout=layout, walk=walk)
+def device_cus():
+    b = lxv.Backend(threads=64)
+    try:
+        return int(re.search(r"(\d+) CUs", b.describe()).group(1))
+    finally:
+        b.close()
+
+
+def run_point(layout, batch, wpc, walk, budget, start, ed_seed, prefix, rounds=1):
+    # lanes = CUs x waves per CU x 64 x rounds: with rounds = 1 exactly the
+    # resident waves, so "waves per CU" is the occupancy of the point
+    s = lxv.GpuSearch(start=start, ed_seed=ed_seed, batch=batch, waves_per_cu=wpc, layout=layout, walk=walk,
+                      threads=device_cus() * wpc * 64 * rounds)
     try:
         b = s.backend
         per_iter = b.threads * b.batch
