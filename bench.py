@@ -263,11 +263,7 @@ def main():
                        "achieved_GBps": value * alg_bytes_per_key / 1e9, "peak_GBps": 8000.0,
                        "frac": value * alg_bytes_per_key / 8e12}
     pmc = load_traffic()
-    wpc = args.waves_per_cu or 16
-    kname = "%s<lxv::%s, %d, %d>" % ("walk_sw_kernel" if args.subwalks else
-                                     ("walk_pair_kernel" if pairs else "walk_kernel"),
-                                    "Field10" if args.layout == lxv.LAYOUT_10X26 else "Field8", kpi,
-                                    4 if wpc >= 16 else (2 if wpc >= 8 else 1))
+    kname, grid_lanes = b.walk_kernel()
     if pmc and kname in pmc.get("kernel", "") and "hbm_bytes_per_key" in pmc:
         # bytes per launch of this run = measured bytes per key x keys per launch
         roofline["traffic"] = pmc["hbm_bytes_per_key"] * per_iter * iters
@@ -291,7 +287,8 @@ def main():
                                 % (args.walk, b.batch, args.budget_log2)) if not args.subwalks else
                                ("sub-walk form: %d affine walks per lane inverted together, single search, 7-hex "
                                 "prefix, 2^%d-key budget" % (args.subwalks, args.budget_log2)),
-                   "threads": b.threads, "keys_per_inversion": kpi, "iters_per_step": iters,
+                   "threads": b.threads, "kernel": kname, "grid_lanes": grid_lanes,
+                   "keys_per_inversion": kpi, "iters_per_step": iters,
                    "prefix": prefix, "layout": "10x25.5" if args.layout == lxv.LAYOUT_10X26 else "8x32",
                    "device": b.describe().strip()},
         "keys": keys, "wall_s": wall, "hits": len(hits), "hits_rechecked": rechecked,

@@ -63,9 +63,10 @@ typedef struct lxv_ctx lxv_ctx;
 typedef struct lxv_open_opts {
     uint32_t size;         /* sizeof(lxv_open_opts) */
     int32_t device;        /* HIP device ordinal; negative means 0 */
-    uint32_t threads;      /* walk lanes; 0 = CUs x waves_per_cu x 64 x 8 (eight rounds of
-                              resident waves, so the dispatcher balances the launch;
-                              sub-walk form: CUs x waves_per_cu x 64) */
+    uint32_t threads;      /* walk lanes; 0 = CUs x waves_per_cu x 64 x R: R = 64 for the
+                              pair walk, whose resident waves take 64-lane groups from a
+                              queue; R = 8 for the projective walk (rounds of waves for
+                              the dispatcher); R = 1 for the sub-walk form */
     uint32_t batch;        /* keys per batched inversion (B); 0 = 64 */
     uint32_t waves_per_cu; /* resident waves per CU the grid is sized for; 0 = 16 */
     uint32_t layout;       /* LXV_LAYOUT_*; 0 (8x32) is the default */
@@ -77,6 +78,9 @@ typedef struct lxv_open_opts {
                               key per thread per iteration) and `threads` above counts
                               lanes */
     uint32_t walk;         /* LXV_WALK_*, for subwalks = 0 */
+    uint32_t resident_lanes; /* pair walk: lanes of the persistent grid, rounded up to 64;
+                              0 = what the device holds at once. With as many or more
+                              than `threads`, one thread walks each lane instead */
 } lxv_open_opts;
 
 /* What one launch needs besides the per-lane state: LaunchArgs,

@@ -55,6 +55,13 @@ int lxv_debug_copy(int device, uint64_t bytes, double* ms);
 int lxv_debug_kernel_attrs(int layout, uint32_t batch, uint32_t waves_per_cu, uint32_t walk, uint32_t* vgprs,
                            uint32_t* sgprs, uint32_t* scratch_bytes, uint32_t* waves_per_simd);
 
+/* The walk kernel a context launches, named as the profiler names it (e.g.
+ * "walk_pair_queue_kernel<lxv::Field8, 64, 4>", NUL-terminated, cut to
+ * name_cap), and the lanes of its grid (= lxv_threads unless the pair walk's
+ * persistent grid is in use, whose resident waves take the lanes in groups
+ * of 64). */
+int lxv_debug_walk_kernel(const lxv_ctx* ctx, char* name, size_t name_cap, uint32_t* grid_lanes);
+
 /* The pair walk (LXV_WALK_PAIRS) of one lane, run by the host build of the
  * device code: the lane's first centre from u_prev_cur (u(k-8), u(k) as 16
  * little-endian words, the lxv_init form) or, when u_prev_cur is NULL, from

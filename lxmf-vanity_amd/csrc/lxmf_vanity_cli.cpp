@@ -41,7 +41,7 @@ using Identity = std::array<uint8_t, 64>;
 constexpr uint32_t kMaxHits = 4096;                     // src/gpu/mod.rs:23
 constexpr double kTargetLaunchS = 0.5;                  // src/gpu/mod.rs:26
 constexpr double kReportIntervalS = 10.0;               // src/main.rs:37
-constexpr uint64_t kLaneStride = 1ull << 40;            // key distance between GPU lanes
+constexpr uint64_t kLaneStride = 1ull << 40;            // key distance between GPU lanes, at most
 constexpr uint32_t kGpuBatch = 256;                     // keys per inversion (the autotuner's pick)
 constexpr uint32_t kGpuWavesPerCu = 16;
 
@@ -99,7 +99,11 @@ class GpuSearch {
             *err = "no system randomness";
             return nullptr;
         }
-        if (lxv_init_seeded(g->ctx_, g->start_, kLaneStride) != LXV_OK) {
+        // lanes x stride stays below 2^63: with many lanes, a smaller stride
+        // (2^24 lanes: 2^38 keys per lane, years of walking at any rate)
+        const uint64_t lanes = lxv_threads(g->ctx_);
+        while (g->stride_ > 1 && lanes > 1 && g->stride_ > (UINT64_MAX >> 1) / lanes) g->stride_ >>= 1;
+        if (lxv_init_seeded(g->ctx_, g->start_, g->stride_) != LXV_OK) {
             *err = lxv_last_error(g->ctx_);
             return nullptr;
         }
@@ -120,7 +124,7 @@ class GpuSearch {
     // The identity for lane t's key number n (src/gpu/mod.rs:139-144).
     Identity identity(uint32_t t, uint64_t n) const {
         Identity id;
-        lxv::step_scalar(start_, (uint64_t)t * kLaneStride + n, id.data());
+        lxv::step_scalar(start_, (uint64_t)t * stride_ + n, id.data());
         std::memcpy(id.data() + 32, seed_, 32);
         return id;
     }
@@ -168,7 +172,7 @@ class GpuSearch {
         for (uint32_t t = 0; t < dump_threads; t++) {
             for (uint64_t i = 0; i < batch; i++) {
                 uint8_t k[32], expected[32], got[32];
-                lxv::step_scalar(start_, (uint64_t)t * kLaneStride + i, k);
+                lxv::step_scalar(start_, (uint64_t)t * stride_ + i, k);
                 lxv::x25519_base(k, expected);
                 const size_t base = (size_t)(t * batch + i) * 8;
                 for (int w = 0; w < 8; w++)
@@ -239,6 +243,7 @@ class GpuSearch {
     lxv_ctx* ctx_ = nullptr;
     uint8_t start_[32] = {0};
     uint8_t seed_[32] = {0};
+    uint64_t stride_ = kLaneStride;
     lxv_launch_args base_{};
 };
 

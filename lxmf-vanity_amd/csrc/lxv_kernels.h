@@ -62,6 +62,12 @@ hipError_t launch_init_sw_ladder(int layout, uint32_t* state, uint64_t* ns, uint
 bool walk_pair_supported(int layout, int batch, int waves);
 hipError_t walk_pair_attributes(int layout, int batch, int waves, hipFuncAttributes* out);
 hipError_t launch_walk_pair(int layout, int batch, int waves, const WalkParams& p, hipStream_t s);
+// The persistent form: p.resident / 64 waves take groups of 64 of the p.lanes
+// lanes from the counter p.queue (zero at launch); the batch arrays are sized
+// for p.resident lanes.
+// Waves of 64 of the persistent kernel one CU holds at once (0: not built).
+int walk_pair_queue_waves_per_cu(int layout, int batch, int waves);
+hipError_t launch_walk_pair_queue(int layout, int batch, int waves, const WalkParams& p, hipStream_t s);
 hipError_t launch_init_pair_from_u(int layout, int batch, uint32_t* state, uint64_t* ns, uint32_t* xor_acc,
                                    const uint32_t* upc, uint32_t lanes, hipStream_t s);
 // base = k_0 / 8 + (B/2 - 1) as 8 little-endian words; lane t starts at

@@ -23,7 +23,13 @@ namespace lxv {
 template <class F, int B, int W>
 __global__ __launch_bounds__(64, W) void walk_pair_kernel(const WalkParams p) {
     const uint32_t lane = blockIdx.x * 64u + threadIdx.x;
-    if (lane < p.lanes) walk_pair_lane<F, B>(p, lane);
+    if (lane < p.lanes) walk_pair_lane<F, B>(p, lane, p.lanes, lane);
+}
+
+// One wave per 64 resident lanes; p.lanes is a multiple of 64.
+template <class F, int B, int W>
+__global__ __launch_bounds__(64, W) void walk_pair_queue_kernel(const WalkParams p) {
+    walk_pair_queue<F, B>(p, blockIdx.x * 64u + threadIdx.x);
 }
 
 template <class F>
@@ -76,6 +82,7 @@ struct PairEntry {
     int batch;
     int waves;
     pair_fn fn;
+    pair_fn queue_fn;  // the persistent form, or null
 };
 
 // One layout's kernels: the walks (B x waves), the per-B initialisers from
@@ -98,8 +105,9 @@ struct PairPart {
 #define LXV_PART_NAME kPairPart10
 #endif
 
-#define LXV_P3(B) {B, 1, walk_pair_kernel<LXV_PF, B, 1>}, {B, 2, walk_pair_kernel<LXV_PF, B, 2>}, \
-                  {B, 4, walk_pair_kernel<LXV_PF, B, 4>}
+#define LXV_PQ(B, W) walk_pair_queue_kernel<LXV_PF, B, W>
+#define LXV_P3(B) {B, 1, walk_pair_kernel<LXV_PF, B, 1>, LXV_PQ(B, 1)}, {B, 2, walk_pair_kernel<LXV_PF, B, 2>, LXV_PQ(B, 2)}, \
+                  {B, 4, walk_pair_kernel<LXV_PF, B, 4>, LXV_PQ(B, 4)}
 static const PairEntry kWalksHere[] = {LXV_P3(16), LXV_P3(32), LXV_P3(64), LXV_P3(128), LXV_P3(256)};
 #undef LXV_P3
 
@@ -122,12 +130,32 @@ static const PairPart* pair_part(int layout) {
     return nullptr;
 }
 
-static pair_fn pair_walk_fn(int layout, int batch, int waves) {
+static const PairEntry* pair_entry(int layout, int batch, int waves) {
     const PairPart* part = pair_part(layout);
     if (!part) return nullptr;
     for (int i = 0; i < part->n_walks; i++)
-        if (part->walks[i].batch == batch && part->walks[i].waves == waves) return part->walks[i].fn;
+        if (part->walks[i].batch == batch && part->walks[i].waves == waves) return &part->walks[i];
     return nullptr;
+}
+
+static pair_fn pair_walk_fn(int layout, int batch, int waves) {
+    const PairEntry* e = pair_entry(layout, batch, waves);
+    return e ? e->fn : nullptr;
+}
+
+int walk_pair_queue_waves_per_cu(int layout, int batch, int waves) {
+    const PairEntry* e = pair_entry(layout, batch, waves);
+    if (!e || !e->queue_fn) return 0;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, e->queue_fn, 64, 0) != hipSuccess) return 0;
+    return blocks;
+}
+
+hipError_t launch_walk_pair_queue(int layout, int batch, int waves, const WalkParams& p, hipStream_t s) {
+    const PairEntry* e = pair_entry(layout, batch, waves);
+    if (!e || !e->queue_fn || p.resident % 64 || p.lanes % 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(e->queue_fn, dim3(p.resident / 64), dim3(64), 0, s, p);
+    return hipGetLastError();
 }
 
 bool walk_pair_supported(int layout, int batch, int waves) { return pair_walk_fn(layout, batch, waves) != nullptr; }

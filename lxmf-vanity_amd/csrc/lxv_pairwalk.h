@@ -182,9 +182,11 @@ LXV_HD void pair_centre_ladder(const uint32_t base[8], uint64_t off, typename F:
 
 #if defined(__HIPCC__)
 // The walk of one lane: `iters` batches of B keys. The centre is the lane's
-// whole state (state slots 0 and 1: x and y).
+// whole state (state slots 0 and 1: x and y); the batch arrays are those of
+// lane mem_lane of mem_lanes.
 template <class F, int B>
-__device__ __forceinline__ void walk_pair_lane(const WalkParams& p, uint32_t lane) {
+__device__ __forceinline__ void walk_pair_lane(const WalkParams& p, uint32_t lane, uint32_t mem_lanes,
+                                               uint32_t mem_lane) {
     typedef typename F::fe fe;
     const uint32_t L = p.lanes;
     const uint64_t n0 = p.ns[lane];
@@ -200,7 +202,7 @@ __device__ __forceinline__ void walk_pair_lane(const WalkParams& p, uint32_t lan
             break;
         }
         const bool dumping = it == 0 && lane < p.dump_threads;
-        pair_batch<F, B>(xc, yc, p.batch_mem, L, lane, [&](int i, const fe& x) {
+        pair_batch<F, B>(xc, yc, p.batch_mem, mem_lanes, mem_lane, [&](int i, const fe& x) {
             uint32_t uw[8];
             F::to_words(x, uw);
             if (dumping) {
@@ -227,6 +229,24 @@ __device__ __forceinline__ void walk_pair_lane(const WalkParams& p, uint32_t lan
     p.xor_acc[2 * L + lane] = x2;
     p.xor_acc[3 * L + lane] = x3;
     atomicAdd(p.batches, (unsigned long long)((n - n0) / B));
+}
+
+// The persistent form of a launch: the grid is only the resident waves
+// (p.resident lanes), and each wave takes groups of 64 lanes from a device
+// counter until every group has walked its batches. Waves that run faster
+// take more groups, so the launch does not end with its slowest wave, and
+// the batch arrays belong to the resident lanes only. Every wave leaves once
+// the counter passes the last group.
+template <class F, int B>
+__device__ __forceinline__ void walk_pair_queue(const WalkParams& p, uint32_t plane) {
+    const uint32_t groups = p.lanes / 64u;
+    for (;;) {
+        uint32_t u = 0;
+        if (__lane_id() == 0) u = atomicAdd(p.queue, 1u);
+        u = __builtin_amdgcn_readfirstlane(__shfl(u, 0));
+        if (u >= groups) break;
+        walk_pair_lane<F, B>(p, u * 64u + __lane_id(), p.resident, plane);
+    }
 }
 #endif
 

@@ -35,10 +35,12 @@ struct WalkParams {
     const uint32_t* stop;        // host-mapped: nonzero asks lanes to stop
     uint32_t* stopped;           // set to 1 if any lane stopped early
     unsigned long long* batches; // batches of B keys completed, all lanes
+    uint32_t* queue;             // device queue counter (persistent pair walk)
     uint32_t max_hits;
     uint32_t dump_threads;
     uint32_t lanes;
     uint32_t iters;
+    uint32_t resident;           // lanes of the persistent grid (its batch arrays' stride)
 };
 
 template <class F>

@@ -42,7 +42,7 @@ class GpuUnavailable(RuntimeError):
 class OpenOpts(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("device", ctypes.c_int32), ("threads", ctypes.c_uint32),
                 ("batch", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32), ("layout", ctypes.c_uint32),
-                ("subwalks", ctypes.c_uint32), ("walk", ctypes.c_uint32)]
+                ("subwalks", ctypes.c_uint32), ("walk", ctypes.c_uint32), ("resident_lanes", ctypes.c_uint32)]
 
 
 class LaunchArgs(ctypes.Structure):
@@ -142,6 +142,7 @@ def lib():
     L.lxv_debug_fe.argtypes = [ctypes.c_int, ctypes.c_int, u32p, u32p, u32p, ctypes.c_uint32, ctypes.c_int]
     L.lxv_debug_address.argtypes = [u32p, u32p, u32p, u32p, ctypes.c_uint32, ctypes.c_int]
     L.lxv_debug_pair_walk.argtypes = [ctypes.c_int, ctypes.c_uint32, u8p, u32p, ctypes.c_uint32, u32p]
+    L.lxv_debug_walk_kernel.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, u32p]
     _lib = L
     return L
 
@@ -224,12 +225,14 @@ class Backend:
     """One GPU context: Backend::{describe, threads, batch, init, launch}."""
 
     def __init__(self, threads=0, batch=64, waves_per_cu=0, layout=LAYOUT_8X32, device=0, subwalks=0,
-                 walk=WALK_PAIRS):
+                 walk=WALK_PAIRS, resident_lanes=0):
         """subwalks=K: K affine sub-walks per lane (threads then counts lanes
         and self.threads is lanes x K; batch is 1 key per thread per launch
-        iteration). walk: WALK_PAIRS or WALK_PROJECTIVE for one walk per lane."""
+        iteration). walk: WALK_PAIRS or WALK_PROJECTIVE for one walk per lane.
+        resident_lanes: the pair walk's persistent grid (0: the device's
+        occupancy; the grid is used only when it is smaller than threads)."""
         o = OpenOpts(ctypes.sizeof(OpenOpts), device, threads, 0 if subwalks else batch, waves_per_cu, layout,
-                     subwalks, walk)
+                     subwalks, walk, resident_lanes)
         h = ctypes.c_void_p()
         rc = lib().lxv_open(ctypes.byref(o), ctypes.byref(h))
         if rc != LXV_OK:
@@ -244,6 +247,13 @@ class Backend:
 
     def describe(self):
         return lib().lxv_describe(self.h).decode()
+
+    def walk_kernel(self):
+        """(name of the walk kernel a launch runs, lanes of its grid)."""
+        buf = ctypes.create_string_buffer(128)
+        grid = ctypes.c_uint32(0)
+        self._check(lib().lxv_debug_walk_kernel(self.h, buf, len(buf), ctypes.byref(grid)), "walk_kernel")
+        return buf.value.decode(), grid.value
 
     def init(self, u_prev_cur):
         arr = (ctypes.c_uint32 * len(u_prev_cur))(*u_prev_cur)
@@ -302,6 +312,14 @@ class Backend:
 
 # ---- GpuSearch (src/gpu/mod.rs:86-207)
 
+def lane_stride(lanes, stride=1 << 40):
+    """The key distance between lanes: stride halved until lanes x stride
+    stays below 2^63 (2^24 lanes: 2^38 keys per lane)."""
+    while stride > 1 and lanes * stride >= 1 << 63:
+        stride >>= 1
+    return stride
+
+
 class GpuSearch:
     """The GPU search over one backend. Lanes start at start + 8 (t x stride)
     (one shared Ed25519 key, as src/gpu/mod.rs:102-104)."""
@@ -314,9 +332,7 @@ class GpuSearch:
         if not start_scalar_ok(self.start):
             raise ValueError("not a start scalar random_start_scalar could return")
         self.seed = ed_seed if ed_seed is not None else os.urandom(32)
-        # threads x stride must stay below 2^63 (many sub-walks: a smaller stride)
-        while stride > 1 and self.backend.threads * stride >= 1 << 63:
-            stride >>= 1
+        stride = lane_stride(self.backend.threads, stride)
         self.stride = stride
         self.backend.init_seeded(self.start, stride)
         self.ed_pub = ed25519_public(self.seed)

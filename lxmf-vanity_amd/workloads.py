@@ -213,9 +213,17 @@ def serving(rate=20.0, duration_s=60.0, policy="short-first", budget_ms=50, seed
 def one_by_one(rate=20.0, duration_s=60.0, seed=4, batch=256, waves_per_cu=16, progress=print):
     """The same arrivals through the plain kernel, one search at a time in
     arrival order, every lane on the current search; launches grow while a
-    launch is short (src/gpu/mod.rs:146-174)."""
+    launch is short (src/gpu/mod.rs:146-174). Every request sets up all
+    lanes anew, so the lanes are eight rounds of the resident waves launched
+    one thread per lane (21 ms of set-up at B=256 on MI355X) rather than the
+    default 64 rounds on the persistent grid (174 ms)."""
     trace = arrivals(rate, duration_s, seed, SERVING_LENGTHS)
-    b = lxv.Backend(batch=batch, waves_per_cu=waves_per_cu)
+    probe = lxv.Backend(batch=batch, waves_per_cu=waves_per_cu)
+    grid, lanes = probe.walk_kernel()[1], probe.threads
+    probe.close()
+    resident = grid if grid < lanes else lanes // 8
+    b = lxv.Backend(batch=batch, waves_per_cu=waves_per_cu, threads=8 * resident, resident_lanes=8 * resident)
+    stride = lxv.lane_stride(b.threads)
     reqs, keys, busy = [], 0, 0.0
     try:
         per_iter = b.threads * b.batch
@@ -229,7 +237,7 @@ def one_by_one(rate=20.0, duration_s=60.0, seed=4, batch=256, waves_per_cu=16, p
                 time.sleep(t_arr - now)
             r = Request(s, digits, t_arr)
             reqs.append(r)
-            b.init_seeded(r.start, lxv.GpuSearch.LANE_STRIDE)
+            b.init_seeded(r.start, stride)
             iters = 1
             while time.perf_counter() - t0 < duration_s:
                 t1 = time.perf_counter()
@@ -240,7 +248,7 @@ def one_by_one(rate=20.0, duration_s=60.0, seed=4, batch=256, waves_per_cu=16, p
                 busy += st.kernel_ms / 1e3
                 if hits:
                     t, n, addr = hits[0]
-                    g = t * lxv.GpuSearch.LANE_STRIDE + n
+                    g = t * stride + n
                     if not r.recheck(g, addr):
                         raise SystemExit("request %d (lane %d, key %d) failed its CPU recheck" % (s, t, n))
                     r.g, r.done = g, time.perf_counter() - t0

@@ -71,9 +71,14 @@ def test_gpu_address_hash_matches_oracle():
 # One walk per lane, 64 keys per inversion, in both limb layouts and both
 # walk forms (the pair walk, the default, and the reference's projective
 # walk), and the sub-walk form (64 affine walks per lane: a thread is one of
-# them, one key per thread per iteration).
+# them, one key per thread per iteration). The "-queue" forms run the pair
+# walk's persistent grid with a few resident waves (4 and 3) taking the 64
+# groups of 64 lanes from the queue; with 4096 lanes the others launch one
+# thread per lane.
 FORMS = {"10x26": dict(threads=4096, batch=64, layout=lxv.LAYOUT_10X26),
          "8x32": dict(threads=4096, batch=64, layout=lxv.LAYOUT_8X32),
+         "8x32-queue": dict(threads=4096, batch=64, layout=lxv.LAYOUT_8X32, resident_lanes=256),
+         "10x26-queue": dict(threads=4096, batch=64, layout=lxv.LAYOUT_10X26, resident_lanes=192),
          "10x26-proj": dict(threads=4096, batch=64, layout=lxv.LAYOUT_10X26, walk=lxv.WALK_PROJECTIVE),
          "8x32-proj": dict(threads=4096, batch=64, layout=lxv.LAYOUT_8X32, walk=lxv.WALK_PROJECTIVE),
          "8x32-b16": dict(threads=4096, batch=16, layout=lxv.LAYOUT_8X32),
@@ -86,6 +91,26 @@ def small_backend(request):
     b = lxv.Backend(**FORMS[request.param])
     yield b
     b.close()
+
+
+def test_walk_kernel_and_grid_of_each_form():
+    def kernel(**kw):
+        b = lxv.Backend(**kw)
+        try:
+            return b.walk_kernel(), b.threads
+        finally:
+            b.close()
+    assert kernel(**FORMS["8x32-queue"]) == (("walk_pair_queue_kernel<lxv::Field8, 64, 4>", 256), 4096)
+    assert kernel(**FORMS["10x26-queue"]) == (("walk_pair_queue_kernel<lxv::Field10, 64, 4>", 192), 4096)
+    assert kernel(**FORMS["8x32"]) == (("walk_pair_kernel<lxv::Field8, 64, 4>", 4096), 4096)
+    assert kernel(**FORMS["8x32-proj"]) == (("walk_kernel<lxv::Field8, 64, 4>", 4096), 4096)
+    # the defaults: a persistent grid of the resident waves (16 per CU at
+    # 128 VGPRs) over 64 times as many lanes
+    (name, grid), lanes = kernel(batch=64)
+    assert name == "walk_pair_queue_kernel<lxv::Field8, 64, 4>"
+    assert grid % (16 * 64) == 0 and lanes == 64 * grid
+    (name, grid), lanes = kernel(batch=64, walk=lxv.WALK_PROJECTIVE)
+    assert name == "walk_kernel<lxv::Field8, 64, 4>" and grid == lanes
 
 
 def test_walk_dump_matches_x25519(small_backend):
@@ -273,12 +298,14 @@ def test_config0_workload_matches_golden(form):
 
 def test_config1_subrange_checksum_matches_oracle():
     # The headline workload's shape (workload seed 1, 7-hex prefix, B = 64,
-    # every lane of the chip at 2^40 keys apart): a 2^28-key sub-range, lanes
-    # 0..L-1 x n < 2^28 / L, walked on the GPU and by the oracle; every hit
-    # and the XOR of all 2^28 addresses must agree.
+    # 2^22 lanes at 2^40 keys apart on the persistent grid of every resident
+    # wave): a 2^28-key sub-range, lanes 0..L-1 x n < 2^28 / L, walked on the
+    # GPU and by the oracle; every hit and the XOR of all 2^28 addresses must
+    # agree.
     start, _, ed_pub, prefix = seeded(1, 7)
-    b = lxv.Backend(batch=64)
+    b = lxv.Backend(batch=64, threads=1 << 22)
     try:
+        assert b.walk_kernel()[0].startswith("walk_pair_queue_kernel")
         per_lane = (1 << 28) // b.threads
         assert per_lane * b.threads == 1 << 28 and per_lane % 64 == 0
         b.init_seeded(start, lxv.GpuSearch.LANE_STRIDE)
