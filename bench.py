@@ -148,6 +148,16 @@ def many_search_main(args):
                "vs_plain": r["keys_per_s"] / base, "kernel_vs_plain": (r["kernel_keys_per_s"] or 0) / base}
         out.update(common)
         out["result"] = r
+    elif args.workload == "one-by-one":
+        r = workloads.one_by_one(duration_s=args.duration_s, progress=log)
+        out = {"metric": "p50/p99 time-to-hit, mixed prefixes (one search at a time, plain kernel)",
+               "value": r["keys_per_s"], "unit": "keys/s", "steps": 0, "warmup": 0, "ms_per_step": None,
+               "p50_s": r["p50_s"], "p99_s": r["p99_s"], "unfinished": r["unfinished"],
+               "config": {"workload": "serving arrivals: Poisson 20/s for %.0f s, 5-9 hex digits "
+                                      "(P=.30/.30/.20/.15/.05), one search at a time in arrival order"
+                                      % args.duration_s, "keys_per_inversion": 256, "layout": "8x32"}}
+        out.update(common)
+        out["result"] = r
     else:
         kw = {"lane_keys_log2": args.lane_keys_log2} if args.lane_keys_log2 else {}
         if args.budget_ms:
@@ -170,7 +180,8 @@ def many_search_main(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("plain", "batched", "serving"), default="plain")
+    ap.add_argument("--workload", choices=("plain", "batched", "serving", "one-by-one"), default="plain",
+                    help="one-by-one: the serving arrivals through the plain kernel alone, one search at a time")
     ap.add_argument("--wall-s", type=float, default=60.0)
     ap.add_argument("--duration-s", type=float, default=60.0)
     ap.add_argument("--policy", choices=("short-first", "fifo"), default="short-first")

@@ -2,8 +2,10 @@
 (x walk form: the pair walk and the reference's projective walk).
 
 Each point opens the backend with that shape (lanes = CUs x waves per CU x 64,
-one resident round, so the waves per CU are the point's occupancy; the
-library's default adds eight rounds of lanes), sets every lane up on the GPU
+one resident round launched one thread per lane, so the waves per CU are the
+point's occupancy; the library's default walks 64 times as many lanes on the
+pair walk's persistent grid, 8 times as many for the projective walk, which
+adds the same few percent to every point), sets every lane up on the GPU
 and walks a fixed budget of keys of one seeded search (an 8-hex-digit prefix
 from workload seed 2), timed with HIP events on the kernel's stream; every hit
 is re-derived on the CPU. Per point it records keys/s, the kernel's VGPRs and
