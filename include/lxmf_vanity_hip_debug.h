@@ -72,6 +72,16 @@ int lxv_debug_walk_kernel(const lxv_ctx* ctx, char* name, size_t name_cap, uint3
 int lxv_debug_pair_walk(int layout, uint32_t batch, const uint8_t start[32], const uint32_t* u_prev_cur,
                         uint32_t batches, uint32_t* out);
 
+/* Lane t's first centre for lxv_init_seeded's pair walk set-up with the
+ * clamped scalar `start`, keys per inversion `batch` and lane stride
+ * `stride`, computed twice by the host build of the device code: out[0..15]
+ * = (x, y) by a full ladder over start / 8 + batch / 2 - 1 + t stride, and
+ * out[16..31] = (x, y) as the set-up kernel computes it, P0 + t S from its
+ * two anchors with a ladder over the `bits` low bits of t (t < 2^bits <=
+ * 2^63). Canonical little-endian words. CPU only. */
+int lxv_debug_pair_centre(int layout, uint32_t batch, const uint8_t start[32], uint64_t stride, uint64_t t,
+                          uint32_t bits, uint32_t* out);
+
 /* The command-line tool's CPU key walk (KeyWalk, src/main.rs:229-290, on
  * the host field arithmetic): the public keys u(start + 8n) for
  * n < batches x 256, as 32-byte little-endian encodings into `out`

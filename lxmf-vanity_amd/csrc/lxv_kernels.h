@@ -71,9 +71,18 @@ hipError_t launch_walk_pair_queue(int layout, int batch, int waves, const WalkPa
 hipError_t launch_init_pair_from_u(int layout, int batch, uint32_t* state, uint64_t* ns, uint32_t* xor_acc,
                                    const uint32_t* upc, uint32_t lanes, hipStream_t s);
 // base = k_0 / 8 + (B/2 - 1) as 8 little-endian words; lane t starts at
-// k_0 + 8 t stride.
+// k_0 + 8 t stride. scratch: kPairInitScratchWords device words for the two
+// anchors and their scalars.
+constexpr uint32_t kPairInitScratchWords = 48;
 hipError_t launch_init_pair_ladder(int layout, uint32_t* state, uint64_t* ns, uint32_t* xor_acc,
-                                   const uint32_t base[8], uint64_t stride, uint32_t lanes, hipStream_t s);
+                                   const uint32_t base[8], uint64_t stride, uint32_t lanes, uint32_t* scratch,
+                                   hipStream_t s);
+// Lane i < n: out[i] = m_i Q for the scalars m_i (8 little-endian words at
+// scalars + i * words_apart, device memory).
+struct PairAnchor;
+hipError_t launch_pair_anchors(const uint32_t* scalars, uint32_t words_apart, uint32_t n, PairAnchor* out,
+                               hipStream_t s);
+uint32_t bit_length(uint64_t v);
 
 struct MultiParams;
 struct MultiInitEntry;
@@ -82,10 +91,12 @@ struct MultiInitEntry;
 bool multi_supported(int layout, int batch, int walk);
 hipError_t launch_multi(int layout, int batch, int walk, const MultiParams& p, hipStream_t s);
 // Sets up the virtual lanes of n admitted searches (entries in device memory)
-// in one launch; max_vlanes bounds every entry's lane count.
+// in one launch; max_vlanes bounds every entry's lane count. The pair walk
+// first computes each entry's anchor into anchors[0..n-1] and takes
+// s_anchor = stride Q (set up once, by launch_pair_anchors).
 hipError_t launch_multi_init(int layout, int walk, uint32_t* pool, uint32_t pool_lanes, const uint32_t q[8],
                              uint64_t stride, const MultiInitEntry* entries, uint32_t n, uint32_t max_vlanes,
-                             hipStream_t s);
+                             PairAnchor* anchors, const PairAnchor* s_anchor, hipStream_t s);
 hipError_t launch_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n,
                            hipStream_t s);
 // lanes must be a multiple of 256; 8 mads per lane per iteration.

@@ -46,7 +46,7 @@ struct SlotDyn {
 // One admitted search's virtual lanes, set up by the ladder in one batched
 // launch per step (block row = entry).
 struct MultiInitEntry {
-    uint32_t base[8];    // as SlotConst::base
+    uint32_t base[8];    // as SlotConst::base (first: the pair walk's anchors read it in place)
     uint32_t state_off;  // first pool lane
     uint32_t vlanes;
     uint32_t pad[2];
@@ -253,13 +253,14 @@ LXV_D void multi_init_lane(uint32_t* pool, uint32_t pool_lanes, uint32_t state_o
     store_state<F>(pool, pool_lanes, lane, 3, z3);
 }
 
-// Virtual lane v of one search set up for the pair walk (base = start / 8 +
-// B/2 - 1): the centre of its first batch, with g = v * stride.
+// Virtual lane v of one search set up for the pair walk: the centre of its
+// first batch, (base + v stride) Q = P0 + v S with P0 = base Q (base = start /
+// 8 + B/2 - 1) and S = stride Q.
 template <class F>
-LXV_D void multi_pair_init_lane(uint32_t* pool, uint32_t pool_lanes, uint32_t state_off, const uint32_t base[8],
-                                uint64_t stride, uint32_t v) {
+LXV_D void multi_pair_init_lane(uint32_t* pool, uint32_t pool_lanes, uint32_t state_off, const PairAnchor& p0,
+                                const PairAnchor& s, uint32_t bits, uint32_t v) {
     typename F::fe xc, yc;
-    pair_centre_ladder<F>(base, (uint64_t)v * stride, xc, yc);
+    pair_centre_offset<F>(p0, s, v, (int)bits, xc, yc);
     store_state<F>(pool, pool_lanes, state_off + v, 0, xc);
     store_state<F>(pool, pool_lanes, state_off + v, 1, yc);
 }

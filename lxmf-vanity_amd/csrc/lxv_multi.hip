@@ -128,6 +128,9 @@ struct lxv_multi {
     SlotDyn* h_dyn = nullptr;
     MultiInitEntry* h_init = nullptr;
     MultiInitEntry* d_init = nullptr;
+    // the pair walk's anchors: [0, max_slots) one admission's P0s, then
+    // S = stride Q, then the stride as 8 words
+    PairAnchor* d_anchor = nullptr;
     hipEvent_t ev_admit = nullptr;
     bool admit_pending = false;
     Unit* h_units = nullptr;
@@ -185,6 +188,7 @@ static void multi_free(lxv_multi* m) {
     (void)hipFree(m->d_batch);
     (void)hipFree(m->d_hits);
     (void)hipFree(m->d_init);
+    (void)hipFree(m->d_anchor);
     if (m->h_slots) (void)hipHostFree(m->h_slots);
     if (m->h_dyn) (void)hipHostFree(m->h_dyn);
     if (m->h_init) (void)hipHostFree(m->h_init);
@@ -289,6 +293,7 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
                            sizeof(uint32_t));
     MALLOC(m->d_hits, (size_t)kMaxHits * sizeof(MultiHit));
     MALLOC(m->d_init, (size_t)m->max_slots * sizeof(MultiInitEntry));
+    MALLOC(m->d_anchor, ((size_t)m->max_slots + 1) * sizeof(PairAnchor) + 8 * sizeof(uint32_t));
 #undef MALLOC
     hipError_t e = hipHostMalloc((void**)&m->h_stop, 64, hipHostMallocMapped | hipHostMallocCoherent);
     if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&m->d_stop, m->h_stop, 0);
@@ -301,6 +306,14 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&m->ev0);
     if (e == hipSuccess) e = hipEventCreate(&m->ev1);
+    if (e == hipSuccess && m->walk == LXV_WALK_PAIRS) {
+        // S = stride Q, shared by every search's lane set-up
+        uint32_t* sw = reinterpret_cast<uint32_t*>(m->d_anchor + m->max_slots + 1);
+        const uint32_t w[8] = {(uint32_t)kStride, (uint32_t)(kStride >> 32), 0, 0, 0, 0, 0, 0};
+        e = hipMemcpy(sw, w, sizeof w, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = launch_pair_anchors(sw, 8, 1, m->d_anchor + m->max_slots, m->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    }
     if (e != hipSuccess) {
         std::string msg = std::string("lxv_multi_open setup: ") + hipGetErrorString(e);
         multi_free(m);
@@ -459,7 +472,7 @@ static int admit(lxv_multi* m) {
         MCHK(m, hipMemcpyAsync(m->d_init, m->h_init, (size_t)n * sizeof(MultiInitEntry), hipMemcpyHostToDevice,
                                m->stream));
         MCHK(m, launch_multi_init(m->layout, m->walk, m->d_pool, m->pool_lanes, m->q, kStride, m->d_init, n, max_vl,
-                                  m->stream));
+                                  m->d_anchor, m->d_anchor + m->max_slots, m->stream));
         MCHK(m, hipEventRecord(m->ev_admit, m->stream));
         m->admit_pending = true;
     }

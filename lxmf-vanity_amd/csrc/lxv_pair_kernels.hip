@@ -5,7 +5,7 @@
 //                     SIMD the registers are budgeted for)
 //   init_pair_from_u  the lane's first centre from host-computed u(k-8), u(k)
 //                     (init_walk, src/gpu/kernel.cu:21-24)
-//   init_pair_ladder  the lane's first centre from an on-device ladder
+//   init_pair_ladder  the lane's first centre from two anchors (pair_anchor)
 //
 // The file is compiled twice, once per limb layout (LXV_PAIR_PART = 8 or 10),
 // so the two halves of the kernel table build in parallel; the launchers are
@@ -59,8 +59,8 @@ __global__ __launch_bounds__(64) void init_pair_from_u_kernel(uint32_t* state, u
 }
 
 struct PairLadderArgs {
-    uint32_t base[8];  // k_0 / 8 + H
-    uint64_t stride;
+    const PairAnchor* anchors;  // [0] P0 = (k_0 / 8 + H) Q, [1] S = stride Q
+    uint32_t bits;              // lanes - 1 < 2^bits
     uint32_t lanes;
 };
 
@@ -70,9 +70,23 @@ __global__ __launch_bounds__(64) void init_pair_ladder_kernel(uint32_t* state, u
     const uint32_t lane = blockIdx.x * 64u + threadIdx.x;
     if (lane >= a.lanes) return;
     typename F::fe xc, yc;
-    pair_centre_ladder<F>(a.base, (uint64_t)lane * a.stride, xc, yc);
+    pair_centre_offset<F>(a.anchors[0], a.anchors[1], lane, (int)a.bits, xc, yc);
     pair_lane_reset<F>(state, ns, xor_acc, a.lanes, lane, xc, yc);
 }
+
+#if LXV_PAIR_PART == 8
+// Lane i < n: out[i] = m_i Q for the scalar m_i at scalars + i * words_apart
+// (canonical words, so one build serves both limb layouts).
+__global__ __launch_bounds__(64) void pair_anchor_kernel(const uint32_t* scalars, uint32_t words_apart, uint32_t n,
+                                                         PairAnchor* out) {
+    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    if (i >= n) return;
+    uint32_t m[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) m[k] = scalars[(size_t)i * words_apart + k];
+    pair_anchor<Field8>(m, out[i]);
+}
+#endif
 
 typedef void (*pair_fn)(const WalkParams);
 typedef void (*pair_init_fn)(uint32_t*, uint64_t*, uint32_t*, const uint32_t*, uint32_t);
@@ -185,14 +199,41 @@ hipError_t launch_init_pair_from_u(int layout, int batch, uint32_t* state, uint6
     return hipGetLastError();
 }
 
+uint32_t bit_length(uint64_t v) {
+    uint32_t b = 0;
+    while (v) {
+        b++;
+        v >>= 1;
+    }
+    return b;
+}
+
+hipError_t launch_pair_anchors(const uint32_t* scalars, uint32_t words_apart, uint32_t n, PairAnchor* out,
+                               hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(pair_anchor_kernel, dim3((n + 63) / 64), dim3(64), 0, s, scalars, words_apart, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_init_pair_ladder(int layout, uint32_t* state, uint64_t* ns, uint32_t* xor_acc,
-                                   const uint32_t base[8], uint64_t stride, uint32_t lanes, hipStream_t s) {
-    PairLadderArgs a;
-    for (int i = 0; i < 8; i++) a.base[i] = base[i];
-    a.stride = stride;
-    a.lanes = lanes;
+                                   const uint32_t base[8], uint64_t stride, uint32_t lanes, uint32_t* scratch,
+                                   hipStream_t s) {
     const PairPart* part = pair_part(layout);
-    if (!part) return hipErrorInvalidValue;
+    if (!part || !lanes) return hipErrorInvalidValue;
+    // the two anchors' scalars (base, stride) in scratch[0..15], the anchors
+    // after them
+    uint32_t m[16] = {0};
+    for (int i = 0; i < 8; i++) m[i] = base[i];
+    m[8] = (uint32_t)stride;
+    m[9] = (uint32_t)(stride >> 32);
+    hipError_t e = hipMemcpy(scratch, m, sizeof m, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    PairLadderArgs a;
+    a.anchors = reinterpret_cast<const PairAnchor*>(scratch + 16);
+    a.bits = bit_length(lanes - 1);
+    a.lanes = lanes;
+    e = launch_pair_anchors(scratch, 8, 2, reinterpret_cast<PairAnchor*>(scratch + 16), s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(part->init_ladder, dim3((lanes + 63) / 64), dim3(64), 0, s, state, ns, xor_acc, a);
     return hipGetLastError();
 }

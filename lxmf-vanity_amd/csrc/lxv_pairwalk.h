@@ -142,15 +142,20 @@ LXV_HD void recover_affine(const typename F::fe& xr, const typename F::fe& zr, c
     x = F::mul(F::mul(F::mul(xr, zr), F::mul(zs, v2)), inv);
 }
 
-// (x, y) + T_i by the affine addition formula (x != u(T_i)).
+// (x, y) + (ut, vt) by the affine addition formula (x != ut).
 template <class F>
-LXV_HD void affine_add_tab(typename F::fe& x, typename F::fe& y, int i) {
+LXV_HD void affine_add(typename F::fe& x, typename F::fe& y, const typename F::fe& ut, const typename F::fe& vt) {
     typedef typename F::fe fe;
-    const fe ut = pair_tab_u<F>(i), vt = pair_tab_v<F>(i);
     const fe l = F::mul(F::sub(vt, y), F::invert(F::sub(ut, x)));
     const fe x2 = F::tight(F::sub(F::sub(F::sq(l), fe_small<F>(kMontA)), F::add(x, ut)));
     y = F::tight(F::sub(F::mul(l, F::sub(x, x2)), y));
     x = x2;
+}
+
+// (x, y) + T_i (x != u(T_i)).
+template <class F>
+LXV_HD void affine_add_tab(typename F::fe& x, typename F::fe& y, int i) {
+    affine_add<F>(x, y, pair_tab_u<F>(i), pair_tab_v<F>(i));
 }
 
 // The centre of a lane's first batch from the reference's start data: u(k - 8)
@@ -178,6 +183,44 @@ LXV_HD void pair_centre_ladder(const uint32_t base[8], uint64_t off, typename F:
     fe x2, z2, x3, z3;
     ladder_mq<F>(base, off, qw, x2, z2, x3, z3);
     recover_affine<F>(x2, z2, x3, z3, pair_tab_u<F>(1), pair_tab_v<F>(1), xc, yc);
+}
+
+// An affine point as canonical little-endian words.
+struct PairAnchor {
+    uint32_t x[8], y[8];
+};
+
+// The anchor mQ of a lane set-up, for a scalar m (8 little-endian words) with
+// mQ and (m + 1)Q not the identity.
+template <class F>
+LXV_HD void pair_anchor(const uint32_t m[8], PairAnchor& out) {
+    typename F::fe x, y;
+    pair_centre_ladder<F>(m, 0, x, y);
+    F::to_words(x, out.x);
+    F::to_words(y, out.y);
+}
+
+// Lane t's first centre (b + t s) Q = P0 + t S from two anchors computed once
+// per set-up: P0 = bQ (lane 0's centre) and S = sQ. A ladder over the `bits`
+// low bits of t (t < 2^bits) on S gives tS and (t + 1)S projectively, the
+// recovery with T = S makes tS affine, and one affine addition adds P0: about
+// a sixth of a full 255-step ladder per lane at 2^24 lanes. The addition needs
+// x(tS) != x(P0), i.e. t s != +-b (mod l). That holds for every set-up here:
+// b = k_0 / 8 + c with k_0 a valid start (2^254 <= k_0 < 2^255 - 2^72, which
+// random_start_scalar and start_scalar_ok guarantee) and |c| < 2^7 lies in
+// [2^251 - 2^7, l - 2^69), while 1 <= t s < 2^63.
+template <class F>
+LXV_HD void pair_centre_offset(const PairAnchor& p0, const PairAnchor& s, uint64_t t, int bits, typename F::fe& xc,
+                               typename F::fe& yc) {
+    typedef typename F::fe fe;
+    xc = F::from_words(p0.x);
+    yc = F::from_words(p0.y);
+    if (t == 0) return;
+    const fe us = F::from_words(s.x), vs = F::from_words(s.y);
+    fe x2, z2, x3, z3, xt, yt;
+    ladder_u64<F>(t, bits, us, x2, z2, x3, z3);
+    recover_affine<F>(x2, z2, x3, z3, us, vs, xt, yt);
+    affine_add<F>(xc, yc, xt, yt);
 }
 
 #if defined(__HIPCC__)

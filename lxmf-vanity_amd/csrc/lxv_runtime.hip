@@ -50,6 +50,7 @@ struct lxv_ctx {
     uint32_t* d_xor = nullptr;
     uint2* d_batch = nullptr;
     uint32_t* d_counters = nullptr;            // [0] hit count, [1] stopped, [2..3] batches (u64), [4] queue
+    uint32_t* d_anchor = nullptr;              // the pair walk's set-up: two scalars and their anchors
     lxv_hit* d_hits = nullptr;
     uint32_t hits_cap = 0;
     uint32_t* d_dump = nullptr;
@@ -89,6 +90,7 @@ static void free_ctx(lxv_ctx* c) {
     (void)hipFree(c->d_xor);
     (void)hipFree(c->d_batch);
     (void)hipFree(c->d_counters);
+    (void)hipFree(c->d_anchor);
     (void)hipFree(c->d_hits);
     (void)hipFree(c->d_dump);
     if (c->h_stop) (void)hipHostFree(c->h_stop);
@@ -213,6 +215,7 @@ extern "C" int lxv_open(const lxv_open_opts* opts, lxv_ctx** out) {
     ALLOC(c->d_ns, L * sizeof(uint64_t));
     ALLOC(c->d_xor, L * 4 * sizeof(uint32_t));
     ALLOC(c->d_counters, 16 * sizeof(uint32_t));
+    ALLOC(c->d_anchor, kPairInitScratchWords * sizeof(uint32_t));
 #undef ALLOC
     if ((e = hipHostMalloc((void**)&c->h_stop, 64, hipHostMallocMapped | hipHostMallocCoherent)) != hipSuccess) {
         std::string m = std::string("hipHostMalloc stop flag: ") + hipGetErrorString(e);
@@ -304,7 +307,7 @@ extern "C" int lxv_init_seeded(lxv_ctx* c, const uint8_t start[32], uint64_t str
     HIPCHK(c, hipSetDevice(c->device));
     if (!c->subwalks && c->walk == LXV_WALK_PAIRS)
         HIPCHK(c, launch_init_pair_ladder((int)c->layout, c->d_state, c->d_ns, c->d_xor, base, stride, c->lanes,
-                                          c->stream));
+                                          c->d_anchor, c->stream));
     else if (c->subwalks)
         HIPCHK(c, launch_init_sw_ladder((int)c->layout, c->d_state, c->d_ns, c->d_xor, base, a.q, stride, c->phys,
                                         c->subwalks, c->stream));
@@ -721,6 +724,38 @@ static bool host_pair_walk_run(int layout, int batch, const uint32_t* upc, const
 }
 
 }  // namespace lxv
+
+template <class F>
+static void host_pair_centre(const uint32_t base[8], uint64_t stride, uint64_t t, int bits, uint32_t* out) {
+    typename F::fe x, y;
+    pair_centre_ladder<F>(base, t * stride, x, y);
+    F::to_words(x, out);
+    F::to_words(y, out + 8);
+    PairAnchor a[2];
+    const uint32_t sw[8] = {(uint32_t)stride, (uint32_t)(stride >> 32), 0, 0, 0, 0, 0, 0};
+    pair_anchor<Field8>(base, a[0]);
+    pair_anchor<Field8>(sw, a[1]);
+    pair_centre_offset<F>(a[0], a[1], t, bits, x, y);
+    F::to_words(x, out + 16);
+    F::to_words(y, out + 24);
+}
+
+extern "C" int lxv_debug_pair_centre(int layout, uint32_t batch, const uint8_t start[32], uint64_t stride, uint64_t t,
+                                     uint32_t bits, uint32_t* out) {
+    if (!out || !start || !start_scalar_ok(start)) return LXV_E_ARG;
+    if (batch < 4 || batch > 256 || (batch & (batch - 1)) || stride == 0 || bits > 63 || (t >> bits) != 0 ||
+        (t && stride > (UINT64_MAX >> 1) / t))
+        return LXV_E_ARG;
+    uint32_t base[8];
+    scalar_div8_plus(start, (int64_t)batch / 2 - 1, base);
+    if (layout == LXV_LAYOUT_8X32)
+        host_pair_centre<Field8>(base, stride, t, (int)bits, out);
+    else if (layout == LXV_LAYOUT_10X26)
+        host_pair_centre<Field10>(base, stride, t, (int)bits, out);
+    else
+        return LXV_E_ARG;
+    return LXV_OK;
+}
 
 extern "C" int lxv_debug_pair_walk(int layout, uint32_t batch, const uint8_t start[32], const uint32_t* upc,
                                    uint32_t batches, uint32_t* out) {

@@ -221,6 +221,45 @@ LXV_D void init_lane_from_u(uint32_t* state, uint64_t* ns, uint32_t* xor_acc, co
     for (int k = 0; k < 4; k++) xor_acc[(size_t)k * lanes + lane] = 0;
 }
 
+// One step of the X25519 ladder (RFC 7748 section 5) over the point with
+// u-coordinate x1, after the conditional swap: (x2:z2) doubles and (x3:z3)
+// becomes their sum.
+template <class F>
+LXV_HD void ladder_step(const typename F::fe& x1, typename F::fe& x2, typename F::fe& z2, typename F::fe& x3,
+                        typename F::fe& z3) {
+    typedef typename F::fe fe;
+    const fe a = F::add(x2, z2), aa = F::sq(a), b = F::sub(x2, z2), bb = F::sq(b);
+    const fe e = F::sub(aa, bb);
+    const fe c = F::add(x3, z3), d = F::sub(x3, z3);
+    const fe da = F::mul(d, a), cb = F::mul(c, b);
+    x3 = F::sq(F::add(da, cb));
+    z3 = F::mul(x1, F::sq(F::sub(da, cb)));
+    x2 = F::mul(aa, bb);
+    z2 = F::mul(e, F::add(aa, F::mul_small(e, 121665)));
+}
+
+// The ladder over the point with u-coordinate x1 for the low `bits` bits of
+// m: (X:Z) of mP and (m + 1)P.
+template <class F>
+LXV_HD void ladder_u64(uint64_t m, int bits, const typename F::fe& x1, typename F::fe& x2, typename F::fe& z2,
+                       typename F::fe& x3, typename F::fe& z3) {
+    x2 = F::one();
+    z2 = F::zero();
+    x3 = x1;
+    z3 = F::one();
+    uint32_t swap = 0;
+    for (int i = bits - 1; i >= 0; i--) {
+        const uint32_t bit = (uint32_t)(m >> i) & 1u;
+        swap ^= bit;
+        F::cswap(x2, x3, swap);
+        F::cswap(z2, z3, swap);
+        swap = bit;
+        ladder_step<F>(x1, x2, z2, x3, z3);
+    }
+    F::cswap(x2, x3, swap);
+    F::cswap(z2, z3, swap);
+}
+
 // The X25519 ladder over Q = u(qw) for the scalar m = base + off (base as 8
 // little-endian words, off < 2^63, no wrap): (X:Z) of mQ and (m + 1)Q.
 template <class F>
@@ -250,14 +289,7 @@ LXV_HD void ladder_mq(const uint32_t base[8], uint64_t off, const uint32_t qw[8]
             F::cswap(x2, x3, swap);
             F::cswap(z2, z3, swap);
             swap = bit;
-            const fe a = F::add(x2, z2), aa = F::sq(a), b = F::sub(x2, z2), bb = F::sq(b);
-            const fe e = F::sub(aa, bb);
-            const fe c = F::add(x3, z3), d = F::sub(x3, z3);
-            const fe da = F::mul(d, a), cb = F::mul(c, b);
-            x3 = F::sq(F::add(da, cb));
-            z3 = F::mul(x1, F::sq(F::sub(da, cb)));
-            x2 = F::mul(aa, bb);
-            z2 = F::mul(e, F::add(aa, F::mul_small(e, 121665)));
+            ladder_step<F>(x1, x2, z2, x3, z3);
         }
     }
     F::cswap(x2, x3, swap);

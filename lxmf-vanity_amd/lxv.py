@@ -143,6 +143,8 @@ def lib():
     L.lxv_debug_address.argtypes = [u32p, u32p, u32p, u32p, ctypes.c_uint32, ctypes.c_int]
     L.lxv_debug_pair_walk.argtypes = [ctypes.c_int, ctypes.c_uint32, u8p, u32p, ctypes.c_uint32, u32p]
     L.lxv_debug_walk_kernel.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, u32p]
+    L.lxv_debug_pair_centre.argtypes = [ctypes.c_int, ctypes.c_uint32, u8p, ctypes.c_uint64, ctypes.c_uint64,
+                                        ctypes.c_uint32, u32p]
     _lib = L
     return L
 
@@ -482,6 +484,18 @@ def debug_address(ed_words, nh_words, u_words, on_gpu):
     if rc != LXV_OK:
         raise RuntimeError("lxv_debug_address failed (%d)" % rc)
     return list(O[:4 * n])
+
+
+def debug_pair_centre(batch, start, stride, t, bits, layout=LAYOUT_8X32):
+    """Lane t's first centre of the lxv_init_seeded pair-walk set-up on the
+    CPU, twice: ((x, y) by a full ladder, (x, y) from the set-up's two
+    anchors), each coordinate as an integer."""
+    out = (ctypes.c_uint32 * 32)()
+    rc = lib().lxv_debug_pair_centre(layout, batch, start, stride, t, bits, out)
+    if rc != LXV_OK:
+        raise RuntimeError("lxv_debug_pair_centre failed (%d)" % rc)
+    v = [sum(out[8 * k + i] << (32 * i) for i in range(8)) for k in range(4)]
+    return (v[0], v[1]), (v[2], v[3])
 
 
 def debug_pair_walk(batch, batches, start=None, u_prev_cur=None, layout=LAYOUT_8X32):

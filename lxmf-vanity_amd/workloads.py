@@ -215,8 +215,8 @@ def one_by_one(rate=20.0, duration_s=60.0, seed=4, batch=256, waves_per_cu=16, p
     arrival order, every lane on the current search; launches grow while a
     launch is short (src/gpu/mod.rs:146-174). Every request sets up all
     lanes anew, so the lanes are eight rounds of the resident waves launched
-    one thread per lane (21 ms of set-up at B=256 on MI355X) rather than the
-    default 64 rounds on the persistent grid (174 ms)."""
+    one thread per lane rather than the default 64 rounds on the persistent
+    grid: an eighth of the set-up per request."""
     trace = arrivals(rate, duration_s, seed, SERVING_LENGTHS)
     probe = lxv.Backend(batch=batch, waves_per_cu=waves_per_cu)
     grid, lanes = probe.walk_kernel()[1], probe.threads

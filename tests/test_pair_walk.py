@@ -9,6 +9,8 @@
   each batch size, both limb layouts and both ways of setting a lane up
   (from u(k - 8), u(k) as Backend::init hands them over, and the ladder of
   lxv_init_seeded).
+* lxv_init_seeded's lane set-up from two anchors (P0 + t S) against the full
+  ladder, the oracle's public keys and the curve equation.
 """
 
 import os
@@ -90,3 +92,36 @@ def test_host_pair_walk_rejects_bad_arguments():
             lxv.debug_pair_walk(batch, 1, start=start)
     with pytest.raises(RuntimeError):
         lxv.debug_pair_walk(64, 1, start=bytes(32))  # not a clamped start scalar
+
+
+@pytest.mark.parametrize("layout", [lxv.LAYOUT_8X32, lxv.LAYOUT_10X26], ids=["8x32", "10x26"])
+@pytest.mark.parametrize("batch", [64, 256])
+def test_lane_setup_from_anchors_matches_the_ladder(layout, batch):
+    # lxv_init_seeded's pair-walk set-up computes lane t's first centre as
+    # P0 + t S from two anchors and a short ladder over t. It must give the
+    # full ladder's point exactly; its x is the public key of the centre's key
+    # (index t stride + B/2 - 1), the point is on the curve, and adding T_1
+    # = Q gives the next key's public key (which fixes the sign of y).
+    tab = table_points()
+    rng = random.Random(batch + layout)
+    start = lxv.seeded_search(11 + batch, 0)[0]
+    H = batch // 2 - 1
+    cases = [(1 << 40, t, 24) for t in (0, 1, 2, 3, 5, (1 << 23) - 1)]
+    cases += [(1 << 38, (1 << 24) - 1, 24), (1, 0, 0), (1, 1, 1), (7, (1 << 32) - 1, 32), (1 << 20, 6, 40)]
+    cases += [(1 << 30, rng.randrange(1 << 32), 32) for _ in range(3)]
+    for stride, t, bits in cases:
+        (x1, y1), (x2, y2) = lxv.debug_pair_centre(batch, start, stride, t, bits, layout=layout)
+        assert (x2, y2) == (x1, y1), (stride, t)
+        assert (y2 * y2 - (x2 ** 3 + A * x2 * x2 + x2)) % P == 0
+        g = t * stride + H
+        u = lambda k: int.from_bytes(orc.x25519(orc.step_scalar(start, k)), "little")  # noqa: E731
+        assert x2 == u(g), (stride, t)
+        assert affine_add((x2, y2), tab[0])[0] == u(g + 1), (stride, t)
+
+
+def test_lane_setup_rejects_bad_arguments():
+    start = lxv.seeded_search(3, 0)[0]
+    for args in [(64, start, 0, 1, 1), (64, start, 1, 2, 1), (64, start, 1, 1, 64), (48, start, 1, 1, 1),
+                 (64, bytes(32), 1, 1, 1), (64, start, 1 << 62, 2, 2)]:
+        with pytest.raises(RuntimeError):
+            lxv.debug_pair_centre(*args)
