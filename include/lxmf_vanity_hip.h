@@ -257,6 +257,12 @@ typedef struct lxv_multi_opts {
     uint32_t walk;          /* LXV_WALK_*: PAIRS (default; 8x32, batch 64 or 256) keeps
                                2 field elements of state per virtual lane,
                                PROJECTIVE 4 */
+    uint32_t pipelined;     /* 1: lxv_multi_step enqueues the next step before it
+                               waits for the one before, so the host's planning and
+                               read-back overlap the walk; a step's answers then come
+                               back from the following call (at the same time on the
+                               clock: the call waits for that step). 0 (default):
+                               each call runs one step to completion */
 } lxv_multi_opts;
 
 typedef struct lxv_search_spec {
@@ -296,7 +302,9 @@ int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out);
 /* Queues a search; returns its id. */
 int lxv_multi_submit(lxv_multi* m, const lxv_search_spec* spec, uint64_t* id);
 /* Admits what fits, relaunches the kernel once over about budget_ms of work,
- * and returns up to cap finished searches (more are kept for the next call). */
+ * and returns up to cap finished searches (more are kept for the next call).
+ * budget_ms = 0 launches nothing; with `pipelined` it collects the step still
+ * in flight, so a caller can drain before it stops stepping. */
 int lxv_multi_step(lxv_multi* m, uint32_t budget_ms, lxv_search_result* out, uint32_t cap, uint32_t* n_out);
 int lxv_multi_cancel(lxv_multi* m, uint64_t id);
 int lxv_multi_stats(lxv_multi* m, lxv_multi_stat* out);

@@ -92,11 +92,11 @@ bool multi_supported(int layout, int batch, int walk);
 hipError_t launch_multi(int layout, int batch, int walk, const MultiParams& p, hipStream_t s);
 // Sets up the virtual lanes of n admitted searches (entries in device memory)
 // in one launch; max_vlanes bounds every entry's lane count. The pair walk
-// first computes each entry's anchor into anchors[0..n-1] and takes
-// s_anchor = stride Q (set up once, by launch_pair_anchors).
+// takes each entry's anchor p0 and s_anchor = stride Q (host memory, passed
+// by value).
 hipError_t launch_multi_init(int layout, int walk, uint32_t* pool, uint32_t pool_lanes, const uint32_t q[8],
                              uint64_t stride, const MultiInitEntry* entries, uint32_t n, uint32_t max_vlanes,
-                             PairAnchor* anchors, const PairAnchor* s_anchor, hipStream_t s);
+                             const PairAnchor* s_anchor, hipStream_t s);
 hipError_t launch_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n,
                            hipStream_t s);
 // lanes must be a multiple of 256; 8 mads per lane per iteration.

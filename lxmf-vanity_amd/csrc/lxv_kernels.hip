@@ -12,6 +12,8 @@
 //                  (lxv_subwalk.h), with its two initialisers
 //   debug kernels  field operations and the address hash on given inputs, for
 //                  parity tests against the CPU
+#include <cstring>
+
 #include "fe10.h"
 #include "fe8.h"
 #include "lxv_kernels.h"
@@ -141,8 +143,7 @@ struct MultiInitArgs {
     uint32_t pool_lanes;
     uint32_t bits;  // pair walk: every entry's vlanes - 1 < 2^bits
     const MultiInitEntry* entries;
-    const PairAnchor* anchors;   // pair walk: entry y's P0
-    const PairAnchor* s_anchor;  // pair walk: stride Q
+    PairAnchor s_anchor;  // pair walk: stride Q
 };
 
 // Block (x, y): lanes 64 x .. 64 x + 63 of admitted search y.
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(64) void multi_pair_init_kernel(uint32_t* pool, con
     const MultiInitEntry& e = a.entries[blockIdx.y];
     const uint32_t v = blockIdx.x * 64u + threadIdx.x;
     if (v < e.vlanes)
-        multi_pair_init_lane<F>(pool, a.pool_lanes, e.state_off, a.anchors[blockIdx.y], *a.s_anchor, a.bits, v);
+        multi_pair_init_lane<F>(pool, a.pool_lanes, e.state_off, e.p0, a.s_anchor, a.bits, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -333,7 +334,7 @@ hipError_t launch_multi(int layout, int batch, int walk, const MultiParams& p, h
 
 hipError_t launch_multi_init(int layout, int walk, uint32_t* pool, uint32_t pool_lanes, const uint32_t q[8],
                              uint64_t stride, const MultiInitEntry* entries, uint32_t n, uint32_t max_vlanes,
-                             PairAnchor* anchors, const PairAnchor* s_anchor, hipStream_t s) {
+                             const PairAnchor* s_anchor, hipStream_t s) {
     if (n == 0) return hipSuccess;
     if (n > 65535 || max_vlanes == 0) return hipErrorInvalidValue;
     MultiInitArgs a;
@@ -342,15 +343,11 @@ hipError_t launch_multi_init(int layout, int walk, uint32_t* pool, uint32_t pool
     a.pool_lanes = pool_lanes;
     a.bits = bit_length(max_vlanes - 1);
     a.entries = entries;
-    a.anchors = anchors;
-    a.s_anchor = s_anchor;
+    std::memset(&a.s_anchor, 0, sizeof a.s_anchor);
     const dim3 grid((max_vlanes + 63) / 64, n), block(64);
     if (walk == LXV_WALK_PAIRS && layout == LXV_LAYOUT_8X32) {
-        if (!anchors || !s_anchor) return hipErrorInvalidValue;
-        // each entry's base is its first 8 words
-        const hipError_t e = launch_pair_anchors(reinterpret_cast<const uint32_t*>(entries),
-                                                 sizeof(MultiInitEntry) / sizeof(uint32_t), n, anchors, s);
-        if (e != hipSuccess) return e;
+        if (!s_anchor) return hipErrorInvalidValue;
+        a.s_anchor = *s_anchor;
         hipLaunchKernelGGL(multi_pair_init_kernel<Field8>, grid, block, 0, s, pool, a);
     } else if (walk == LXV_WALK_PAIRS) {
         return hipErrorInvalidValue;

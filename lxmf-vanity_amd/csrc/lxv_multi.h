@@ -46,10 +46,11 @@ struct SlotDyn {
 // One admitted search's virtual lanes, set up by the ladder in one batched
 // launch per step (block row = entry).
 struct MultiInitEntry {
-    uint32_t base[8];    // as SlotConst::base (first: the pair walk's anchors read it in place)
+    uint32_t base[8];    // as SlotConst::base
     uint32_t state_off;  // first pool lane
     uint32_t vlanes;
     uint32_t pad[2];
+    PairAnchor p0;       // the pair walk: base Q
 };
 
 struct Unit {

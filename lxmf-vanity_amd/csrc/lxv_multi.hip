@@ -36,11 +36,17 @@ struct Search {
     uint8_t start[32];
     SlotConst sc;
     uint32_t vlanes = 0, slot = kNone, state_off = 0;
-    // Progress: blocks [0, next_block) are walked completely, and virtual
-    // waves [0, wave_cursor) of block next_block as well.
+    // Planned progress: once every launched step has run, blocks
+    // [0, next_block) are walked completely, and virtual waves
+    // [0, wave_cursor) of block next_block as well. done_block / done_cursor:
+    // the same for the steps collected so far; after_block / after_cursor:
+    // the planned progress at the end of the step launched with parity p.
     uint32_t next_block = 0, wave_cursor = 0;
-    // This step's units: virtual waves [w0, w1) of blocks [next_block, + nblocks).
+    uint32_t done_block = 0, done_cursor = 0;
+    uint32_t after_block[2] = {0, 0}, after_cursor[2] = {0, 0};
+    // The step being planned: virtual waves [w0, w1) of blocks [next_block, + nblocks).
     uint32_t w0 = 0, w1 = 0, nblocks = 0;
+    PairAnchor p0;  // the pair walk: (start / 8 + B/2 - 1) Q, from the host at submission
     // Best hit so far in walk order (block, n, v); final once its block is whole.
     bool hit = false;
     uint32_t hit_block = 0, hit_vlane = 0, hit_addr[4] = {0, 0, 0, 0};
@@ -103,10 +109,26 @@ uint32_t be_word(const uint8_t* b) {
 
 }  // namespace
 
+// What a launched step walked: (search id, slot) of every search with units
+// in it, so its hits and progress are credited to the searches that owned
+// the slots when it was planned.
+struct StepRec {
+    std::vector<std::pair<uint64_t, uint32_t>> members;
+};
+
 struct lxv_multi {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t rstream = nullptr;  // read-back of a finished step while the next one runs
+    // Steps alternate between two parities, each with its own events, hit
+    // buffer, counters, unit staging and graph, so that with `pipelined` the
+    // next step is planned and enqueued before the last one is collected.
+    hipEvent_t ev0[2] = {nullptr, nullptr}, ev1[2] = {nullptr, nullptr};
+    bool pipelined = false;
+    bool inflight = false;  // a launched step not yet collected, of parity 1 - parity
+    int parity = 0;         // of the next step to launch
+    StepRec rec[2];
+    PairAnchor s_anchor;    // the pair walk: stride Q, shared by every search
     std::string err = "no error";
     int layout = LXV_LAYOUT_8X32, batch = 256, walk = LXV_WALK_PAIRS;
     uint32_t rounds = 1, limbs = 8;
@@ -117,10 +139,10 @@ struct lxv_multi {
     SlotDyn* d_dyn = nullptr;
     Unit* d_units = nullptr;
     uint32_t* d_nunits = nullptr;
-    uint32_t* d_counters = nullptr;
+    uint32_t* d_counters = nullptr;  // 2 x 16 words, one block per parity
     uint32_t* d_pool = nullptr;
     uint2* d_batch = nullptr;
-    MultiHit* d_hits = nullptr;
+    MultiHit* d_hits = nullptr;      // 2 x kMaxHits, one half per parity
     uint32_t* h_stop = nullptr;
     uint32_t* d_stop = nullptr;
     // pinned staging: slot tables mirrored on the host, admission entries
@@ -128,19 +150,17 @@ struct lxv_multi {
     SlotDyn* h_dyn = nullptr;
     MultiInitEntry* h_init = nullptr;
     MultiInitEntry* d_init = nullptr;
-    // the pair walk's anchors: [0, max_slots) one admission's P0s, then
-    // S = stride Q, then the stride as 8 words
-    PairAnchor* d_anchor = nullptr;
     hipEvent_t ev_admit = nullptr;
     bool admit_pending = false;
-    Unit* h_units = nullptr;
-    uint32_t* h_nunits = nullptr;
+    Unit* h_units[2] = {nullptr, nullptr};
+    uint32_t* h_nunits[2] = {nullptr, nullptr};
     uint32_t units_cap = 0;
     MultiHit* h_hits = nullptr;
-    // graph
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    MultiParams params;
+    uint32_t* h_counters = nullptr;  // pinned, 16 words
+    // graphs, one per parity
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    MultiParams params[2];
     // host bookkeeping
     std::deque<uint64_t> queue;
     std::map<uint64_t, Search> searches;
@@ -177,8 +197,10 @@ static void multi_free(lxv_multi* m) {
     if (!m) return;
     (void)hipSetDevice(m->device);
     if (m->stream) (void)hipStreamSynchronize(m->stream);
-    if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
-    if (m->graph) (void)hipGraphDestroy(m->graph);
+    for (int p = 0; p < 2; p++) {
+        if (m->gexec[p]) (void)hipGraphExecDestroy(m->gexec[p]);
+        if (m->graph[p]) (void)hipGraphDestroy(m->graph[p]);
+    }
     (void)hipFree(m->d_slots);
     (void)hipFree(m->d_dyn);
     (void)hipFree(m->d_units);
@@ -188,18 +210,21 @@ static void multi_free(lxv_multi* m) {
     (void)hipFree(m->d_batch);
     (void)hipFree(m->d_hits);
     (void)hipFree(m->d_init);
-    (void)hipFree(m->d_anchor);
     if (m->h_slots) (void)hipHostFree(m->h_slots);
     if (m->h_dyn) (void)hipHostFree(m->h_dyn);
     if (m->h_init) (void)hipHostFree(m->h_init);
     if (m->ev_admit) (void)hipEventDestroy(m->ev_admit);
     if (m->h_stop) (void)hipHostFree(m->h_stop);
-    if (m->h_units) (void)hipHostFree(m->h_units);
-    if (m->h_nunits) (void)hipHostFree(m->h_nunits);
+    for (int p = 0; p < 2; p++) {
+        if (m->h_units[p]) (void)hipHostFree(m->h_units[p]);
+        if (m->h_nunits[p]) (void)hipHostFree(m->h_nunits[p]);
+        if (m->ev0[p]) (void)hipEventDestroy(m->ev0[p]);
+        if (m->ev1[p]) (void)hipEventDestroy(m->ev1[p]);
+    }
     if (m->h_hits) (void)hipHostFree(m->h_hits);
-    if (m->ev0) (void)hipEventDestroy(m->ev0);
-    if (m->ev1) (void)hipEventDestroy(m->ev1);
+    if (m->h_counters) (void)hipHostFree(m->h_counters);
     if (m->stream) (void)hipStreamDestroy(m->stream);
+    if (m->rstream) (void)hipStreamDestroy(m->rstream);
     delete m;
 }
 
@@ -235,6 +260,7 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
     m->layout = (int)o.layout;
     m->walk = (int)o.walk;
     m->use_graph = !o.no_graph;
+    m->pipelined = o.pipelined != 0;
     if (o.max_searches) m->max_slots = o.max_searches;
     if (o.max_vlanes) m->max_vlanes = std::max<uint32_t>(64, o.max_vlanes / 64 * 64);
     if (o.lane_keys_log2) {
@@ -287,32 +313,33 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
     MALLOC(m->d_slots, (size_t)m->max_slots * sizeof(SlotConst));
     MALLOC(m->d_dyn, (size_t)m->max_slots * sizeof(SlotDyn));
     MALLOC(m->d_nunits, 64);
-    MALLOC(m->d_counters, 64);
+    MALLOC(m->d_counters, 2 * 16 * sizeof(uint32_t));
     MALLOC(m->d_pool, (size_t)m->pool_lanes * lane_bytes);
     MALLOC(m->d_batch, (size_t)m->phys_lanes * (m->walk == LXV_WALK_PAIRS ? m->batch / 2 : m->batch * 2) * m->limbs *
                            sizeof(uint32_t));
-    MALLOC(m->d_hits, (size_t)kMaxHits * sizeof(MultiHit));
+    MALLOC(m->d_hits, 2 * (size_t)kMaxHits * sizeof(MultiHit));
     MALLOC(m->d_init, (size_t)m->max_slots * sizeof(MultiInitEntry));
-    MALLOC(m->d_anchor, ((size_t)m->max_slots + 1) * sizeof(PairAnchor) + 8 * sizeof(uint32_t));
 #undef MALLOC
     hipError_t e = hipHostMalloc((void**)&m->h_stop, 64, hipHostMallocMapped | hipHostMallocCoherent);
     if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&m->d_stop, m->h_stop, 0);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_nunits, 64, hipHostMallocDefault);
+    for (int p = 0; p < 2 && e == hipSuccess; p++) e = hipHostMalloc((void**)&m->h_nunits[p], 64, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_slots, (size_t)m->max_slots * sizeof(SlotConst), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_dyn, (size_t)m->max_slots * sizeof(SlotDyn), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_init, (size_t)m->max_slots * sizeof(MultiInitEntry), hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev_admit, hipEventDisableTiming);
     if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_hits, (size_t)kMaxHits * sizeof(MultiHit), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_counters, 64, hipHostMallocDefault);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreate(&m->ev0);
-    if (e == hipSuccess) e = hipEventCreate(&m->ev1);
-    if (e == hipSuccess && m->walk == LXV_WALK_PAIRS) {
-        // S = stride Q, shared by every search's lane set-up
-        uint32_t* sw = reinterpret_cast<uint32_t*>(m->d_anchor + m->max_slots + 1);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->rstream, hipStreamNonBlocking);
+    for (int p = 0; p < 2 && e == hipSuccess; p++) {
+        e = hipEventCreate(&m->ev0[p]);
+        if (e == hipSuccess) e = hipEventCreate(&m->ev1[p]);
+    }
+    if (m->walk == LXV_WALK_PAIRS) {
+        // S = stride Q, shared by every search's lane set-up (host build of
+        // the device code)
         const uint32_t w[8] = {(uint32_t)kStride, (uint32_t)(kStride >> 32), 0, 0, 0, 0, 0, 0};
-        e = hipMemcpy(sw, w, sizeof w, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = launch_pair_anchors(sw, 8, 1, m->d_anchor + m->max_slots, m->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+        pair_anchor<Field8>(w, m->s_anchor);
     }
     if (e != hipSuccess) {
         std::string msg = std::string("lxv_multi_open setup: ") + hipGetErrorString(e);
@@ -320,22 +347,25 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
         return mfail(nullptr, LXV_E_HIP, msg);
     }
     *(volatile uint32_t*)m->h_stop = 0;
-    std::memset(&m->params, 0, sizeof m->params);
-    if (m->layout == LXV_LAYOUT_8X32)
-        q_limbs<Field8>(m->q, m->params.qp, m->params.qm);
-    else
-        q_limbs<Field10>(m->q, m->params.qp, m->params.qm);
-    m->params.slots = m->d_slots;
-    m->params.dyn = m->d_dyn;
-    m->params.n_units = m->d_nunits;
-    m->params.counters = m->d_counters;
-    m->params.state = m->d_pool;
-    m->params.batch_mem = m->d_batch;
-    m->params.hits = m->d_hits;
-    m->params.stop = m->d_stop;
-    m->params.pool_lanes = m->pool_lanes;
-    m->params.phys_lanes = m->phys_lanes;
-    m->params.max_hits = kMaxHits;
+    for (int p = 0; p < 2; p++) {
+        MultiParams& mp = m->params[p];
+        std::memset(&mp, 0, sizeof mp);
+        if (m->layout == LXV_LAYOUT_8X32)
+            q_limbs<Field8>(m->q, mp.qp, mp.qm);
+        else
+            q_limbs<Field10>(m->q, mp.qp, mp.qm);
+        mp.slots = m->d_slots;
+        mp.dyn = m->d_dyn;
+        mp.n_units = m->d_nunits;
+        mp.counters = m->d_counters + 16 * p;
+        mp.state = m->d_pool;
+        mp.batch_mem = m->d_batch;
+        mp.hits = m->d_hits + (size_t)kMaxHits * p;
+        mp.stop = m->d_stop;
+        mp.pool_lanes = m->pool_lanes;
+        mp.phys_lanes = m->phys_lanes;
+        mp.max_hits = kMaxHits;
+    }
     *out = m;
     return LXV_OK;
 }
@@ -391,6 +421,7 @@ extern "C" int lxv_multi_submit(lxv_multi* m, const lxv_search_spec* spec, uint6
     s.sc.stride = kStride;
     s.sc.vlanes = s.vlanes;
     s.sc.rounds = m->rounds;
+    if (m->walk == LXV_WALK_PAIRS) pair_anchor<Field8>(s.sc.base, s.p0);
     s.submit_s = m->now();
     m->searches[s.id] = s;
     m->queue.push_back(s.id);
@@ -463,6 +494,7 @@ static int admit(lxv_multi* m) {
                                m->stream));
         MultiInitEntry& e = m->h_init[n++];
         std::memcpy(e.base, s.sc.base, sizeof e.base);
+        e.p0 = s.p0;
         e.state_off = off;
         e.vlanes = s.vlanes;
         e.pad[0] = e.pad[1] = 0;
@@ -472,32 +504,219 @@ static int admit(lxv_multi* m) {
         MCHK(m, hipMemcpyAsync(m->d_init, m->h_init, (size_t)n * sizeof(MultiInitEntry), hipMemcpyHostToDevice,
                                m->stream));
         MCHK(m, launch_multi_init(m->layout, m->walk, m->d_pool, m->pool_lanes, m->q, kStride, m->d_init, n, max_vl,
-                                  m->d_anchor, m->d_anchor + m->max_slots, m->stream));
+                                  &m->s_anchor, m->stream));
         MCHK(m, hipEventRecord(m->ev_admit, m->stream));
         m->admit_pending = true;
     }
     return LXV_OK;
 }
 
-static int launch_step(lxv_multi* m) {
+static int launch_step(lxv_multi* m, int p) {
+    uint32_t* counters = m->d_counters + 16 * p;
     if (m->use_graph) {
-        if (!m->gexec) {
+        if (!m->gexec[p]) {
+            // capture on an idle stream (once per parity, and after the unit
+            // list moves)
+            MCHK(m, hipStreamSynchronize(m->stream));
             MCHK(m, hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
-            hipError_t e = launch_clear(m->d_counters, 16, m->stream);
-            if (e == hipSuccess) e = launch_multi(m->layout, m->batch, m->walk, m->params, m->stream);
-            hipError_t e2 = hipStreamEndCapture(m->stream, &m->graph);
+            hipError_t e = launch_clear(counters, 16, m->stream);
+            if (e == hipSuccess) e = launch_multi(m->layout, m->batch, m->walk, m->params[p], m->stream);
+            hipError_t e2 = hipStreamEndCapture(m->stream, &m->graph[p]);
             if (e != hipSuccess) return mfail(m, LXV_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
             MCHK(m, e2);
-            MCHK(m, hipGraphInstantiate(&m->gexec, m->graph, nullptr, nullptr, 0));
+            MCHK(m, hipGraphInstantiate(&m->gexec[p], m->graph[p], nullptr, nullptr, 0));
         }
-        MCHK(m, hipEventRecord(m->ev0, m->stream));
-        MCHK(m, hipGraphLaunch(m->gexec, m->stream));
-        MCHK(m, hipEventRecord(m->ev1, m->stream));
+        MCHK(m, hipEventRecord(m->ev0[p], m->stream));
+        MCHK(m, hipGraphLaunch(m->gexec[p], m->stream));
+        MCHK(m, hipEventRecord(m->ev1[p], m->stream));
     } else {
-        MCHK(m, launch_clear(m->d_counters, 16, m->stream));
-        MCHK(m, hipEventRecord(m->ev0, m->stream));
-        MCHK(m, launch_multi(m->layout, m->batch, m->walk, m->params, m->stream));
-        MCHK(m, hipEventRecord(m->ev1, m->stream));
+        MCHK(m, launch_clear(counters, 16, m->stream));
+        MCHK(m, hipEventRecord(m->ev0[p], m->stream));
+        MCHK(m, launch_multi(m->layout, m->batch, m->walk, m->params[p], m->stream));
+        MCHK(m, hipEventRecord(m->ev1[p], m->stream));
+    }
+    return LXV_OK;
+}
+
+// Plans about budget_ms of work from the planned progress and launches it
+// with parity p; *launched says whether anything was. Each active search may
+// use at most its lanes' worth of the chip for the budget. Priorities are
+// served in ascending order; searches of one priority share what is left
+// equally (water-filling), so a search may get nothing this step and keep its
+// state for later. A search whose share is at least a block walks whole
+// blocks; otherwise it walks that many virtual waves of its current block,
+// and the block completes over the next steps.
+static int plan_step(lxv_multi* m, uint32_t budget_ms, int p, bool* launched) {
+    *launched = false;
+    m->rec[p].members.clear();
+    std::vector<Search*> act;
+    for (auto& kv : m->searches)
+        if (kv.second.active) act.push_back(&kv.second);
+    if (act.empty()) return LXV_OK;
+    const double budget_s = budget_ms / 1e3;
+    const double wave_keys = 64.0 * m->rounds * m->batch;          // one virtual wave, one block
+    const double lane_keys = budget_s * m->rate / m->phys_lanes;  // one lane, busy for the budget
+    std::vector<std::pair<double, Search*>> want;
+    for (Search* s : act)
+        want.push_back({(double)std::min(s->vlanes, m->phys_lanes) * std::max(lane_keys, (double)m->rounds * m->batch), s});
+    std::sort(want.begin(), want.end(), [](const std::pair<double, Search*>& a, const std::pair<double, Search*>& b) {
+        if (a.second->priority != b.second->priority) return a.second->priority < b.second->priority;
+        return a.first != b.first ? a.first < b.first : a.second->id < b.second->id;
+    });
+    double left = m->rate * budget_s;
+    size_t n_units = 0;
+    for (size_t i = 0; i < want.size(); i++) {
+        Search* s = want[i].second;
+        size_t tier_end = i;
+        while (tier_end < want.size() && want[tier_end].second->priority == s->priority) tier_end++;
+        const double share = std::min(want[i].first, left / (double)(tier_end - i));
+        left -= share;
+        const uint32_t vwaves = s->vlanes / 64;
+        const double block_keys = wave_keys * vwaves;
+        if (share < 0.5 * wave_keys) {
+            s->w0 = s->w1 = s->wave_cursor;  // nothing this step
+            s->nblocks = 0;
+        } else if (s->wave_cursor == 0 && share >= block_keys && !s->hit) {
+            s->w0 = 0;
+            s->w1 = vwaves;
+            s->nblocks = (uint32_t)std::floor(share / block_keys);
+        } else {
+            const uint32_t k = (uint32_t)std::max(1.0, std::round(share / wave_keys));
+            s->w0 = s->wave_cursor;
+            s->w1 = std::min(vwaves, s->wave_cursor + k);
+            s->nblocks = 1;
+        }
+        n_units += s->w1 - s->w0;
+    }
+    if (n_units == 0) {
+        // shares below one unit each: the first search in service order
+        // still walks one virtual wave, so every step makes progress
+        Search* s = want[0].second;
+        s->w0 = s->wave_cursor;
+        s->w1 = s->wave_cursor + 1;
+        s->nblocks = 1;
+        n_units = 1;
+    }
+    if (n_units > m->units_cap) {
+        // the staging and device unit lists may still be read by a step in
+        // flight: let it finish before they are replaced
+        MCHK(m, hipStreamSynchronize(m->stream));
+        for (int q = 0; q < 2; q++) {
+            if (m->h_units[q]) (void)hipHostFree(m->h_units[q]);
+            m->h_units[q] = nullptr;
+        }
+        (void)hipFree(m->d_units);
+        m->d_units = nullptr;
+        m->units_cap = 0;
+        const size_t capn = std::max<size_t>(n_units, 4096);
+        for (int q = 0; q < 2; q++) MCHK(m, hipHostMalloc((void**)&m->h_units[q], capn * sizeof(Unit), hipHostMallocDefault));
+        MCHK(m, hipMalloc((void**)&m->d_units, capn * sizeof(Unit)));
+        m->units_cap = (uint32_t)capn;
+        for (int q = 0; q < 2; q++) {
+            m->params[q].units = m->d_units;
+            if (m->gexec[q]) {
+                (void)hipGraphExecDestroy(m->gexec[q]);
+                (void)hipGraphDestroy(m->graph[q]);
+                m->gexec[q] = nullptr;
+                m->graph[q] = nullptr;
+            }
+        }
+    }
+    // Longest units first, so the tail of the launch is short units.
+    std::sort(act.begin(), act.end(), [](const Search* a, const Search* b) {
+        return a->nblocks != b->nblocks ? a->nblocks > b->nblocks : a->id < b->id;
+    });
+    size_t k = 0;
+    for (Search* s : act) {
+        if (s->w1 > s->w0) m->rec[p].members.push_back({s->id, s->slot});
+        for (uint32_t w = s->w0; w < s->w1; w++) m->h_units[p][k++] = Unit{s->slot, w, s->next_block, s->nblocks};
+        // the planned progress once this step has run
+        const uint32_t vwaves = s->vlanes / 64;
+        if (s->w1 > s->w0) {
+            if (s->w0 == 0 && s->w1 == vwaves) {
+                s->next_block += s->nblocks;  // whole blocks (a hit stops units only past its block)
+            } else {
+                s->wave_cursor = s->w1;
+                if (s->wave_cursor == vwaves) {
+                    s->wave_cursor = 0;
+                    s->next_block += 1;
+                }
+            }
+        }
+        s->after_block[p] = s->next_block;
+        s->after_cursor[p] = s->wave_cursor;
+    }
+    *m->h_nunits[p] = (uint32_t)k;
+    MCHK(m, hipMemcpyAsync(m->d_units, m->h_units[p], k * sizeof(Unit), hipMemcpyHostToDevice, m->stream));
+    MCHK(m, hipMemcpyAsync(m->d_nunits, m->h_nunits[p], sizeof(uint32_t), hipMemcpyHostToDevice, m->stream));
+    const int rc = launch_step(m, p);
+    if (rc) return rc;
+    *launched = true;
+    return LXV_OK;
+}
+
+// Waits for the step launched with parity p and credits it: keys, the rate
+// estimate, every hit to the search that owned its slot when the step was
+// planned (searches finished since are skipped), the completed progress, and
+// the searches whose answer is final or whose key budget is spent.
+static int collect_step(lxv_multi* m, int p) {
+    MCHK(m, hipEventSynchronize(m->ev1[p]));
+    float ms = 0;
+    MCHK(m, hipEventElapsedTime(&ms, m->ev0[p], m->ev1[p]));
+    // on the read-back stream, so a step enqueued since is not waited for
+    MCHK(m, hipMemcpyAsync(m->h_counters, m->d_counters + 16 * p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           m->rstream));
+    MCHK(m, hipStreamSynchronize(m->rstream));
+    const uint64_t kkeys = (uint64_t)m->h_counters[2] | (uint64_t)m->h_counters[3] << 32;
+    const uint32_t nh = m->h_counters[1];
+    if (nh > kMaxHits) return mfail(m, LXV_E_STATE, "lxv_multi_step: hit buffer overflow");
+    if (nh) {
+        MCHK(m, hipMemcpyAsync(m->h_hits, m->d_hits + (size_t)kMaxHits * p, (size_t)nh * sizeof(MultiHit),
+                               hipMemcpyDeviceToHost, m->rstream));
+        MCHK(m, hipStreamSynchronize(m->rstream));
+    }
+    m->steps++;
+    m->keys += kkeys;
+    m->busy_s += ms / 1e3;
+    m->last_ms = ms;
+    if (ms > 1.0 && kkeys > 0) m->rate = 0.5 * m->rate + 0.5 * (kkeys / (ms / 1e3));
+    // the searches this step walked, by slot, if they still hold it
+    std::vector<Search*> by_slot(m->max_slots, nullptr);
+    std::vector<Search*> walked;
+    for (const auto& mem : m->rec[p].members) {
+        auto it = m->searches.find(mem.first);
+        if (it == m->searches.end() || !it->second.active || it->second.slot != mem.second) continue;
+        by_slot[mem.second] = &it->second;
+        walked.push_back(&it->second);
+    }
+    // Per search, keep the lowest hit in walk order: block, then n, then v.
+    for (uint32_t i = 0; i < nh; i++) {
+        const MultiHit& h = m->h_hits[i];
+        if (h.slot >= m->max_slots) return mfail(m, LXV_E_STATE, "lxv_multi_step: hit for a slot out of range");
+        Search* s = by_slot[h.slot];
+        if (!s) continue;  // its search has finished since the step was planned
+        if (!s->hit || h.block < s->hit_block ||
+            (h.block == s->hit_block && (h.n < s->hit_n || (h.n == s->hit_n && h.vlane < s->hit_vlane)))) {
+            s->hit = true;
+            s->hit_block = h.block;
+            s->hit_n = h.n;
+            s->hit_vlane = h.vlane;
+            std::memcpy(s->hit_addr, h.address, sizeof s->hit_addr);
+        }
+    }
+    const double block_of = (double)m->rounds * m->batch;  // keys per lane per block
+    for (Search* s : walked) {
+        s->done_block = s->after_block[p];
+        s->done_cursor = s->after_cursor[p];
+        if (s->hit && s->done_block > s->hit_block) {
+            // every block up to the hit's is walked: the answer is final
+            s->keys = (uint64_t)(s->hit_block + 1) * s->vlanes * (uint64_t)block_of;
+            finish(m, *s, 1, (uint64_t)s->hit_vlane * kStride + s->hit_n, s->hit_addr);
+            continue;
+        }
+        s->keys = (uint64_t)s->done_block * s->vlanes * (uint64_t)block_of +
+                  (uint64_t)s->done_cursor * 64 * (uint64_t)block_of;
+        if (s->max_keys && s->keys >= s->max_keys) finish(m, *s, 0, 0, nullptr);
     }
     return LXV_OK;
 }
@@ -508,143 +727,24 @@ extern "C" int lxv_multi_step(lxv_multi* m, uint32_t budget_ms, lxv_search_resul
     MCHK(m, hipSetDevice(m->device));
     int rc = admit(m);
     if (rc) return rc;
-    // Plan about budget_ms of work. Each active search may use at most its
-    // lanes' worth of the chip for the budget. Priorities are served in
-    // ascending order; searches of one priority share what is left equally
-    // (water-filling), so a search may get nothing this step and keep its
-    // state for later. A search whose share is at least a block walks whole
-    // blocks; otherwise it walks that many virtual waves of its current block,
-    // and the block completes over the next steps.
-    std::vector<Search*> act;
-    for (auto& kv : m->searches)
-        if (kv.second.active) act.push_back(&kv.second);
-    if (!act.empty()) {
-        const double budget_s = std::max(1u, budget_ms) / 1e3;
-        const double wave_keys = 64.0 * m->rounds * m->batch;          // one virtual wave, one block
-        const double lane_keys = budget_s * m->rate / m->phys_lanes;  // one lane, busy for the budget
-        std::vector<std::pair<double, Search*>> want;
-        for (Search* s : act)
-            want.push_back({(double)std::min(s->vlanes, m->phys_lanes) * std::max(lane_keys, (double)m->rounds * m->batch), s});
-        std::sort(want.begin(), want.end(), [](const std::pair<double, Search*>& a, const std::pair<double, Search*>& b) {
-            if (a.second->priority != b.second->priority) return a.second->priority < b.second->priority;
-            return a.first != b.first ? a.first < b.first : a.second->id < b.second->id;
-        });
-        double left = m->rate * budget_s;
-        size_t n_units = 0;
-        for (size_t i = 0; i < want.size(); i++) {
-            Search* s = want[i].second;
-            size_t tier_end = i;
-            while (tier_end < want.size() && want[tier_end].second->priority == s->priority) tier_end++;
-            const double share = std::min(want[i].first, left / (double)(tier_end - i));
-            left -= share;
-            const uint32_t vwaves = s->vlanes / 64;
-            const double block_keys = wave_keys * vwaves;
-            if (share < 0.5 * wave_keys) {
-                s->w0 = s->w1 = s->wave_cursor;  // nothing this step
-                s->nblocks = 0;
-            } else if (s->wave_cursor == 0 && share >= block_keys && !s->hit) {
-                s->w0 = 0;
-                s->w1 = vwaves;
-                s->nblocks = (uint32_t)std::floor(share / block_keys);
-            } else {
-                const uint32_t k = (uint32_t)std::max(1.0, std::round(share / wave_keys));
-                s->w0 = s->wave_cursor;
-                s->w1 = std::min(vwaves, s->wave_cursor + k);
-                s->nblocks = 1;
-            }
-            n_units += s->w1 - s->w0;
-        }
-        if (n_units == 0) {
-            // shares below one unit each: the first search in service order
-            // still walks one virtual wave, so every step makes progress
-            Search* s = want[0].second;
-            s->w0 = s->wave_cursor;
-            s->w1 = s->wave_cursor + 1;
-            s->nblocks = 1;
-            n_units = 1;
-        }
-        if (n_units > m->units_cap) {
-            if (m->h_units) (void)hipHostFree(m->h_units);
-            (void)hipFree(m->d_units);
-            m->h_units = nullptr;
-            m->d_units = nullptr;
-            m->units_cap = 0;
-            const size_t capn = std::max<size_t>(n_units, 4096);
-            MCHK(m, hipHostMalloc((void**)&m->h_units, capn * sizeof(Unit), hipHostMallocDefault));
-            MCHK(m, hipMalloc((void**)&m->d_units, capn * sizeof(Unit)));
-            m->units_cap = (uint32_t)capn;
-            m->params.units = m->d_units;
-            if (m->gexec) {
-                (void)hipGraphExecDestroy(m->gexec);
-                (void)hipGraphDestroy(m->graph);
-                m->gexec = nullptr;
-                m->graph = nullptr;
-            }
-        }
-        // Longest units first, so the tail of the launch is short units.
-        std::sort(act.begin(), act.end(), [](const Search* a, const Search* b) {
-            return a->nblocks != b->nblocks ? a->nblocks > b->nblocks : a->id < b->id;
-        });
-        size_t k = 0;
-        for (Search* s : act)
-            for (uint32_t w = s->w0; w < s->w1; w++) m->h_units[k++] = Unit{s->slot, w, s->next_block, s->nblocks};
-        *m->h_nunits = (uint32_t)k;
-        MCHK(m, hipMemcpyAsync(m->d_units, m->h_units, k * sizeof(Unit), hipMemcpyHostToDevice, m->stream));
-        MCHK(m, hipMemcpyAsync(m->d_nunits, m->h_nunits, sizeof(uint32_t), hipMemcpyHostToDevice, m->stream));
-        rc = launch_step(m);
+    const int p = m->parity;
+    bool launched = false;
+    if (budget_ms) {  // 0: launch nothing (with `pipelined`: only collect the step in flight)
+        rc = plan_step(m, budget_ms, p, &launched);
         if (rc) return rc;
-        MCHK(m, hipEventSynchronize(m->ev1));
-        float ms = 0;
-        MCHK(m, hipEventElapsedTime(&ms, m->ev0, m->ev1));
-        uint32_t counters[4];
-        MCHK(m, hipMemcpy(counters, m->d_counters, sizeof counters, hipMemcpyDeviceToHost));
-        const uint64_t kkeys = (uint64_t)counters[2] | (uint64_t)counters[3] << 32;
-        const uint32_t nh = counters[1];
-        if (nh > kMaxHits) return mfail(m, LXV_E_STATE, "lxv_multi_step: hit buffer overflow");
-        if (nh) MCHK(m, hipMemcpy(m->h_hits, m->d_hits, (size_t)nh * sizeof(MultiHit), hipMemcpyDeviceToHost));
-        m->steps++;
-        m->keys += kkeys;
-        m->busy_s += ms / 1e3;
-        m->last_ms = ms;
-        if (ms > 1.0 && kkeys > 0) m->rate = 0.5 * m->rate + 0.5 * (kkeys / (ms / 1e3));
-        // Per search, keep the lowest hit in walk order: block, then n, then v.
-        std::vector<Search*> by_slot(m->max_slots, nullptr);
-        for (Search* s : act) by_slot[s->slot] = s;
-        for (uint32_t i = 0; i < nh; i++) {
-            const MultiHit& h = m->h_hits[i];
-            Search* s = h.slot < m->max_slots ? by_slot[h.slot] : nullptr;
-            if (!s) return mfail(m, LXV_E_STATE, "lxv_multi_step: hit for a slot with no search");
-            if (!s->hit || h.block < s->hit_block ||
-                (h.block == s->hit_block && (h.n < s->hit_n || (h.n == s->hit_n && h.vlane < s->hit_vlane)))) {
-                s->hit = true;
-                s->hit_block = h.block;
-                s->hit_n = h.n;
-                s->hit_vlane = h.vlane;
-                std::memcpy(s->hit_addr, h.address, sizeof s->hit_addr);
-            }
+    }
+    if (m->pipelined) {
+        // the step launched last call ran meanwhile; collect it while this
+        // one runs
+        if (m->inflight) {
+            rc = collect_step(m, 1 - p);
+            if (rc) return rc;
         }
-        const double block_of = (double)m->rounds * m->batch;  // keys per lane per block
-        for (Search* s : act) {
-            const uint32_t vwaves = s->vlanes / 64;
-            if (s->w0 == 0 && s->w1 == vwaves) {
-                s->next_block += s->nblocks;  // whole blocks (a hit stops units only past its block)
-            } else {
-                s->wave_cursor = s->w1;
-                if (s->wave_cursor == vwaves) {
-                    s->wave_cursor = 0;
-                    s->next_block += 1;
-                }
-            }
-            if (s->hit && s->next_block > s->hit_block) {
-                // every block up to the hit's is walked: the answer is final
-                s->keys = (uint64_t)(s->hit_block + 1) * s->vlanes * (uint64_t)block_of;
-                finish(m, *s, 1, (uint64_t)s->hit_vlane * kStride + s->hit_n, s->hit_addr);
-                continue;
-            }
-            s->keys = (uint64_t)s->next_block * s->vlanes * (uint64_t)block_of +
-                      (uint64_t)s->wave_cursor * 64 * (uint64_t)block_of;
-            if (s->max_keys && s->keys >= s->max_keys) finish(m, *s, 0, 0, nullptr);
-        }
+        m->inflight = launched;
+        if (launched) m->parity = 1 - p;
+    } else if (launched) {
+        rc = collect_step(m, p);
+        if (rc) return rc;
     }
     uint32_t n = 0;
     while (out && n < cap && !m->ready.empty()) {

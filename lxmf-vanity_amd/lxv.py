@@ -65,7 +65,7 @@ class MultiOpts(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("device", ctypes.c_int32), ("max_searches", ctypes.c_uint32),
                 ("batch", ctypes.c_uint32), ("layout", ctypes.c_uint32), ("no_graph", ctypes.c_uint32),
                 ("pool_bytes", ctypes.c_uint64), ("max_vlanes", ctypes.c_uint32), ("lane_keys_log2", ctypes.c_uint32),
-                ("walk", ctypes.c_uint32)]
+                ("walk", ctypes.c_uint32), ("pipelined", ctypes.c_uint32)]
 
 
 class SearchSpec(ctypes.Structure):
@@ -402,12 +402,14 @@ class Multi:
 
     A result is a dict: id, tag, g (key = start + 8 g), address (16 bytes),
     keys, vlanes, status (1 found, 0 gave up, -1 cancelled), submit_s,
-    admit_s, done_s (seconds since open)."""
+    admit_s, done_s (seconds since open). pipelined: each step() enqueues
+    the next step before collecting the previous one, whose answers it
+    returns."""
 
     def __init__(self, max_searches=0, batch=256, layout=LAYOUT_8X32, no_graph=False, pool_bytes=0,
-                 max_vlanes=0, lane_keys_log2=0, device=0, walk=WALK_PAIRS):
+                 max_vlanes=0, lane_keys_log2=0, device=0, walk=WALK_PAIRS, pipelined=False):
         o = MultiOpts(ctypes.sizeof(MultiOpts), device, max_searches, batch, layout, 1 if no_graph else 0,
-                      pool_bytes, max_vlanes, lane_keys_log2, walk)
+                      pool_bytes, max_vlanes, lane_keys_log2, walk, 1 if pipelined else 0)
         h = ctypes.c_void_p()
         if lib().lxv_multi_open(ctypes.byref(o), ctypes.byref(h)) != LXV_OK:
             raise GpuUnavailable(lib().lxv_multi_last_error(None).decode())

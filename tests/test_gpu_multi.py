@@ -3,7 +3,9 @@
 A search with V virtual lanes walks keys start + 8 (v * 2^40 + n); its answer
 must be its first hit in walk order (lowest block of B keys, then lowest
 (n, v)), bit-exact with the oracle walking the same lanes, and the same
-whether the search runs alone or among others, eagerly or by graph replay.
+whether the search runs alone or among others, eagerly or by graph replay,
+one step at a time or pipelined (the next step enqueued before the previous
+one is collected).
 """
 
 import pytest
@@ -45,10 +47,11 @@ def oracle_first_hit(start, ed_pub, prefix, vlanes, upto_block):
     return best[3], best[4]
 
 
-@pytest.mark.parametrize("walk", [lxv.WALK_PAIRS, lxv.WALK_PROJECTIVE], ids=["pairs", "projective"])
-def test_answers_are_first_hits_in_walk_order(walk):
+@pytest.mark.parametrize("walk,pipelined", [(lxv.WALK_PAIRS, False), (lxv.WALK_PROJECTIVE, False),
+                                             (lxv.WALK_PAIRS, True)], ids=["pairs", "projective", "pairs-pipelined"])
+def test_answers_are_first_hits_in_walk_order(walk, pipelined):
     searches = workload(range(100, 112), [2, 3] * 6)
-    res, st = run(searches, walk=walk)
+    res, st = run(searches, walk=walk, pipelined=pipelined)
     for i, (start, ed_pub, prefix) in enumerate(searches):
         r = res[i]
         assert r["status"] == 1
@@ -69,8 +72,9 @@ def test_same_answer_alone_and_in_a_mix():
     # priorities change who is admitted and served first, never an answer
     prio = [i % 4 for i in range(10)] + [5, 5, 5] + [i % 3 for i in range(10)]
     ranked, _ = run(others[:10] + targets + others[10:], budget_ms=5, priorities=prio)
+    piped, _ = run(others[:10] + targets + others[10:], budget_ms=5, priorities=prio, pipelined=True)
     for i, a in enumerate(alone):
-        for res in (mixed, ranked):
+        for res in (mixed, ranked, piped):
             m = res[10 + i]
             assert (m["g"], m["address"]) == (a["g"], a["address"])
 
@@ -82,12 +86,14 @@ def test_blocks_split_across_steps_keep_the_answer():
     # oracle, and `keys` must count every block up to the answer's.
     searches = workload(range(700, 704), [6] * 4)
     split, st = run(searches, budget_ms=1, lane_keys_log2=8)
+    piped, st_p = run(searches, budget_ms=1, lane_keys_log2=8, pipelined=True)
     whole, _ = run(searches, budget_ms=2000, lane_keys_log2=8)
-    assert st["steps"] > 8
+    assert st["steps"] > 8 and st_p["steps"] > 8
     for i, (start, ed_pub, prefix) in enumerate(searches):
         r = split[i]
         assert r["vlanes"] == 65536 and r["status"] == 1
         assert (r["g"], r["address"]) == (whole[i]["g"], whole[i]["address"])
+        assert (piped[i]["g"], piped[i]["address"], piped[i]["keys"]) == (r["g"], r["address"], r["keys"])
         block = (r["g"] % STRIDE) // B
         assert r["keys"] == (block + 1) * r["vlanes"] * B
         assert (r["g"], r["address"]) == oracle_first_hit(start, ed_pub, prefix, r["vlanes"], block)
@@ -102,12 +108,14 @@ def test_graph_replay_matches_eager_launches():
         assert (g_res[i]["g"], g_res[i]["address"]) == (e_res[i]["g"], e_res[i]["address"])
 
 
-@pytest.mark.parametrize("walk,lane_bytes", [(lxv.WALK_PAIRS, 64), (lxv.WALK_PROJECTIVE, 128)],
-                         ids=["pairs", "projective"])
-def test_admission_is_bounded_by_device_memory(walk, lane_bytes):
+@pytest.mark.parametrize("walk,lane_bytes,pipelined", [(lxv.WALK_PAIRS, 64, False), (lxv.WALK_PROJECTIVE, 128, False),
+                                                       (lxv.WALK_PAIRS, 64, True)],
+                         ids=["pairs", "projective", "pairs-pipelined"])
+def test_admission_is_bounded_by_device_memory(walk, lane_bytes, pipelined):
     # room for three 64-lane searches (8 x 32-bit limbs: 2 or 4 field
-    # elements of state per lane)
-    m = lxv.Multi(pool_bytes=3 * 64 * lane_bytes, walk=walk)
+    # elements of state per lane); finished searches hand their slots and
+    # lanes to queued ones, also while a pipelined step is still running
+    m = lxv.Multi(pool_bytes=3 * 64 * lane_bytes, walk=walk, pipelined=pipelined)
     try:
         searches = workload(range(500, 510), [3] * 10)
         for i, (st, ed, pf) in enumerate(searches):
@@ -127,14 +135,17 @@ def test_admission_is_bounded_by_device_memory(walk, lane_bytes):
         m.close()
 
 
-def test_give_up_and_cancel():
-    m = lxv.Multi()
+@pytest.mark.parametrize("pipelined", [False, True], ids=["stepped", "pipelined"])
+def test_give_up_and_cancel(pipelined):
+    # b is cancelled after its first step was launched (pipelined: while it
+    # still runs); a gives up after about 2^22 keys
+    m = lxv.Multi(pipelined=pipelined)
     try:
         (s1, e1, _), (s2, e2, _) = workload([600, 601], [0, 0])
         a = m.submit(s1, e1, "0123456789", max_keys=1 << 22)
         b = m.submit(s2, e2, "fedcba9876")
+        res = m.step(20)
         m.cancel(b)
-        res = []
         for _ in range(200):
             res += m.step(20)
             if m.stats()["resident"] == 0 and m.stats()["queued"] == 0:
