@@ -331,3 +331,23 @@ def test_self_test_and_run():
         assert addr.hex().startswith("abc") and tries > 0
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("resident", [0, 64], ids=["thread-per-lane", "queue"])
+def test_ragged_lane_count_and_empty_launch(resident):
+    # 100 lanes asked for: the library rounds up to whole waves (128), walks
+    # every one of them, and a launch of zero batches walks nothing and
+    # leaves every lane where it was.
+    b = lxv.Backend(threads=100, batch=64, resident_lanes=resident)
+    try:
+        assert b.threads == 128
+        start, _, ed_pub, prefix = seeded(61, 2)
+        b.init_seeded(start, STRIDE)
+        hits, _ = b.launch(lxv.launch_args_for(ed_pub, prefix, iters=0))
+        assert hits == [] and b.stats().keys == 0 and b.read_ns() == [0] * 128
+        hits, _ = b.launch(lxv.launch_args_for(ed_pub, prefix, iters=2, max_hits=1 << 12))
+        want, xor = orc.scan_ranges(start, ed_pub, [(t * STRIDE, 128) for t in range(128)], prefix, cap=1 << 12)
+        assert sorted((t * STRIDE + n, a) for t, n, a in hits) == want
+        assert b.checksum() == xor and b.read_ns() == [128] * 128
+    finally:
+        b.close()
