@@ -270,10 +270,12 @@ def test_stop_ends_a_long_launch_and_the_walk_resumes(small_backend):
 
 @pytest.mark.parametrize("form", [dict(threads=65536, batch=256, layout=lxv.LAYOUT_8X32),
                                   dict(threads=65536, batch=256, layout=lxv.LAYOUT_10X26),
+                                  dict(threads=65536, batch=256, layout=lxv.LAYOUT_8X32, resident_lanes=16384),
+                                  dict(threads=65536, batch=256, layout=lxv.LAYOUT_10X26, resident_lanes=12288),
                                   dict(threads=65536, batch=256, layout=lxv.LAYOUT_8X32, walk=lxv.WALK_PROJECTIVE),
                                   dict(threads=65536, batch=256, layout=lxv.LAYOUT_10X26, walk=lxv.WALK_PROJECTIVE),
                                   dict(threads=1024, subwalks=64)],
-                         ids=["8x32", "10x26", "8x32-proj", "10x26-proj", "8x32-sw64"])
+                         ids=["8x32", "10x26", "8x32-queue", "10x26-queue", "8x32-proj", "10x26-proj", "8x32-sw64"])
 def test_config0_workload_matches_golden(form):
     # The small workload at full size: 2^24 consecutive keys of workload
     # seed 0 split evenly over the threads (thread t walks g in [m t, m t + m)
