@@ -206,8 +206,8 @@ def main():
                     help="also run one plain copy kernel of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build walks on a single MI355X and has no multi-GPU path "
-              "(a search shards by disjoint lane ranges with no exchange; see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU: the library walks on a single MI355X and has no multi-GPU path "
+              "(a search would shard by disjoint lane ranges with no exchange).", file=sys.stderr)
         sys.exit(2)
     if args.workload != "plain":
         many_search_main(args)
