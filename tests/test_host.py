@@ -46,6 +46,16 @@ def test_open_without_gpu_fails_cleanly():
         pytest.skip("a GPU may be present")
     with pytest.raises(lxv.GpuUnavailable):
         lxv.Backend()
+    for kw in ({}, {"pipelined": True}):
+        with pytest.raises(lxv.GpuUnavailable, match="no HIP device"):
+            lxv.Multi(**kw)
+
+
+def test_multi_options_are_checked_before_the_device():
+    with pytest.raises(lxv.GpuUnavailable, match="lane_keys_log2"):
+        lxv.Multi(lane_keys_log2=30)
+    with pytest.raises(lxv.GpuUnavailable, match="no batched kernel"):
+        lxv.Multi(batch=128)
 
 
 def test_host_crypto_known_answers():
