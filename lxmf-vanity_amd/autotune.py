@@ -1,11 +1,12 @@
 """Autotuner of the walk kernel: keys per inversion x waves per CU x limb layout
 (x walk form: the pair walk and the reference's projective walk).
 
-Each point opens the backend with that shape (lanes = CUs x waves per CU x 64,
-one resident round launched one thread per lane, so the waves per CU are the
-point's occupancy; the library's default walks 64 times as many lanes on the
-pair walk's persistent grid, 8 times as many for the projective walk, which
-adds the same few percent to every point), sets every lane up on the GPU
+Each point opens the backend with that shape at the library's own launch
+shape (--shape grid, the default): the pair walk on a persistent grid of
+exactly `waves per CU` resident waves per CU taking 64 groups of 64 lanes
+each, the projective walk one thread per lane over 8 rounds of that many
+waves. --shape round launches one resident round (lanes = CUs x waves per CU
+x 64, one thread per lane) instead. It sets every lane up on the GPU
 and walks a fixed budget of keys of one seeded search (an 8-hex-digit prefix
 from workload seed 2), timed with HIP events on the kernel's stream; every hit
 is re-derived on the CPU. Per point it records keys/s, the kernel's VGPRs and
@@ -64,11 +65,18 @@ def device_cus():
         b.close()
 
 
-def run_point(layout, batch, wpc, walk, budget, start, ed_seed, prefix, rounds=1):
-    # lanes = CUs x waves per CU x 64 x rounds: with rounds = 1 exactly the
-    # resident waves, so "waves per CU" is the occupancy of the point
+def run_point(layout, batch, wpc, walk, budget, start, ed_seed, prefix, shape="grid"):
+    # one resident round = CUs x waves per CU x 64 lanes: "waves per CU" is
+    # the occupancy of the point in either shape
+    resident = device_cus() * wpc * 64
+    if shape == "round":
+        lanes, grid = resident, 0
+    elif walk == lxv.WALK_PAIRS:
+        lanes, grid = 64 * resident, resident
+    else:
+        lanes, grid = 8 * resident, 0
     s = lxv.GpuSearch(start=start, ed_seed=ed_seed, batch=batch, waves_per_cu=wpc, layout=layout, walk=walk,
-                      threads=device_cus() * wpc * 64 * rounds)
+                      threads=lanes, resident_lanes=grid)
     try:
         b = s.backend
         per_iter = b.threads * b.batch
@@ -85,13 +93,15 @@ def run_point(layout, batch, wpc, walk, budget, start, ed_seed, prefix, rounds=1
         for t, n, addr in hits:
             if lxv.address_of_identity(s.identity(t, n)) != addr or not addr.hex().startswith(prefix):
                 raise SystemExit("autotune: hit (lane %d, key %d) failed its CPU recheck" % (t, n))
-        return {"lanes": b.threads, "keys": keys, "kernel_ms": ms, "keys_per_s": keys / (ms / 1e3),
-                "hits": len(hits), "device": b.describe().strip()}
+        kname, grid_lanes = b.walk_kernel()
+        return {"lanes": b.threads, "grid_lanes": grid_lanes, "cus": resident // (wpc * 64), "kernel": kname,
+                "keys": keys, "kernel_ms": ms,
+                "keys_per_s": keys / (ms / 1e3), "hits": len(hits), "device": b.describe().strip()}
     finally:
         s.close()
 
 
-def sweep(budget, walks=WALKS):
+def sweep(budget, walks=WALKS, shape="grid"):
     start, ed_seed, prefix = lxv.seeded_search(2, 8)
     points = []
     for walk, wname in walks:
@@ -99,7 +109,7 @@ def sweep(budget, walks=WALKS):
             for batch in BATCHES:
                 for wpc in WAVES:
                     t0 = time.time()
-                    r = run_point(layout, batch, wpc, walk, budget, start, ed_seed, prefix)
+                    r = run_point(layout, batch, wpc, walk, budget, start, ed_seed, prefix, shape)
                     r.update({"walk": wname, "walk_id": walk, "layout": lname, "layout_id": layout, "batch": batch,
                               "waves_per_cu": wpc})
                     attrs = kernel_attrs(layout, batch, wpc, walk)
@@ -109,14 +119,16 @@ def sweep(budget, walks=WALKS):
                     print("%-10s %-8s B=%-3d waves/CU=%-2d  %.3e keys/s  vgprs=%s scratch=%s  (%.1f s)"
                           % (wname, lname, batch, wpc, r["keys_per_s"], r.get("vgprs"), r.get("scratch_bytes"),
                              time.time() - t0), flush=True)
-    return {"workload": "autotune sweep: single seeded search (seed 2), 8-hex prefix %s, 2^%d keys per point"
-                        % (prefix, budget.bit_length() - 1),
+    return {"workload": "autotune sweep: single seeded search (seed 2), 8-hex prefix %s, 2^%d keys per point, "
+                        "%s launch shape" % (prefix, budget.bit_length() - 1, shape),
             "points": points}
 
 
 def kernel_key(name, grid):
-    # walk_pair_kernel<lxv::Field8, 64, 4> / walk_kernel<...> + lanes identifies a point
-    for kname, wname in (("walk_pair_kernel<", "pairs"), ("walk_kernel<", "projective")):
+    # the kernel's name (walk_pair_queue_kernel<lxv::Field8, 64, 4>, ...) and
+    # its grid identify a point
+    for kname, wname in (("walk_pair_queue_kernel<", "pairs"), ("walk_pair_kernel<", "pairs"),
+                         ("walk_kernel<", "projective")):
         if kname in name:
             inner = name.split(kname, 1)[1].split(">", 1)[0]
             field, batch, waves = [x.strip() for x in inner.split(",")]
@@ -133,11 +145,12 @@ def merge_pmc(table, csv_path):
         d = acc.setdefault(k, {})
         d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
     for p in table["points"]:
-        k = (p.get("walk", "projective"), p["layout"], p["batch"], p.get("waves_per_simd", 0), p["lanes"])
+        grid = p.get("grid_lanes", p["lanes"])
+        k = (p.get("walk", "projective"), p["layout"], p["batch"], p.get("waves_per_simd", 0), grid)
         d = acc.get(k)
         if not d or not d.get("GRBM_GUI_ACTIVE"):
             continue
-        cus = p["lanes"] // (p["waves_per_cu"] * 64)
+        cus = p.get("cus") or p["lanes"] // (p["waves_per_cu"] * 64)
         cycles = d["GRBM_GUI_ACTIVE"] / XCDS
         p["valu_busy_pct"] = 100.0 * d["SQ_ACTIVE_INST_VALU"] * 4 / (cus * SIMDS_PER_CU * cycles)
         if "SQ_INSTS_VALU" in d:
@@ -174,6 +187,8 @@ def main():
     ap.add_argument("--out", default=os.path.join(HERE, "tuned.json"))
     ap.add_argument("--merge-pmc", default=None, help="PMC csv of a profiled pass to merge into --out")
     ap.add_argument("--walk", choices=("all", "pairs", "projective"), default="all")
+    ap.add_argument("--shape", choices=("grid", "round"), default="grid",
+                    help="grid: the library's launch shape at each point's occupancy; round: one resident round")
     args = ap.parse_args()
     if args.merge_pmc:
         table = load(args.out)
@@ -182,7 +197,7 @@ def main():
         table = merge_pmc(table, args.merge_pmc)
     else:
         walks = [w for w in WALKS if args.walk in ("all", w[1])]
-        table = pick(sweep(1 << args.budget_log2, walks))
+        table = pick(sweep(1 << args.budget_log2, walks, args.shape))
     if args.out != "/dev/null":
         with open(args.out, "w") as f:
             json.dump(table, f, indent=1)

@@ -1,0 +1,45 @@
+"""CPU checks of the autotuner's bookkeeping: kernel names from the profiler
+mapped back to sweep points, the VALU-busy merge and the pick."""
+
+import csv
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "lxmf-vanity_amd"))
+import autotune  # noqa: E402
+
+
+def test_kernel_names_map_to_points():
+    k = autotune.kernel_key("void lxv::walk_pair_queue_kernel<lxv::Field8, 64, 4>(lxv::WalkParams)", "262144")
+    assert k == ("pairs", "8x32", 64, 4, 262144)
+    k = autotune.kernel_key("void lxv::walk_pair_kernel<lxv::Field10, 256, 2>(lxv::WalkParams)", "131072")
+    assert k == ("pairs", "10x25.5", 256, 2, 131072)
+    k = autotune.kernel_key("void lxv::walk_kernel<lxv::Field8, 16, 1>(lxv::WalkParams)", "65536")
+    assert k == ("projective", "8x32", 16, 1, 65536)
+    assert autotune.kernel_key("lxv::clear_kernel(unsigned int*, unsigned int)", "256") is None
+
+
+def test_merge_and_pick(tmp_path):
+    pts = [dict(walk="pairs", walk_id=0, layout="8x32", layout_id=0, batch=64, waves_per_cu=16, waves_per_simd=4,
+                lanes=1 << 24, grid_lanes=1 << 18, cus=256, keys_per_s=9.2e9),
+           dict(walk="pairs", walk_id=0, layout="8x32", layout_id=0, batch=256, waves_per_cu=16, waves_per_simd=4,
+                lanes=1 << 24, grid_lanes=1 << 18, cus=256, keys_per_s=9.6e9),
+           dict(walk="projective", walk_id=1, layout="8x32", layout_id=0, batch=64, waves_per_cu=16,
+                waves_per_simd=4, lanes=1 << 21, grid_lanes=1 << 21, cus=256, keys_per_s=6.9e9)]
+    table = autotune.pick({"points": pts})
+    assert table["pick"]["batch"] == 256 and table["pick"]["walk"] == "pairs"
+    assert table["pick_by_batch"]["64"]["keys_per_s"] == 9.2e9
+    path = tmp_path / "pmc.csv"
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Grid_Size", "Counter_Name", "Counter_Value"])
+        w.writeheader()
+        name = "void lxv::walk_pair_queue_kernel<lxv::Field8, 256, 4>(lxv::WalkParams)"
+        # one SIMD-cycle in four issuing VALU work on 256 CUs x 4 SIMDs, per XCD clock
+        for c, v in (("GRBM_GUI_ACTIVE", 8e6), ("SQ_ACTIVE_INST_VALU", 0.25 * 1024 * 1e6 / 4),
+                     ("SQ_INSTS_VALU", 123.0)):
+            w.writerow({"Kernel_Name": name, "Grid_Size": str(1 << 18), "Counter_Name": c, "Counter_Value": v})
+    table = autotune.merge_pmc(table, str(path))
+    merged = [p for p in table["points"] if "valu_busy_pct" in p]
+    assert len(merged) == 1 and merged[0]["batch"] == 256
+    assert abs(merged[0]["valu_busy_pct"] - 25.0) < 1e-9 and merged[0]["valu_insts_per_wave_key"] == 123.0
