@@ -353,3 +353,28 @@ def test_ragged_lane_count_and_empty_launch(resident):
         assert b.checksum() == xor and b.read_ns() == [128] * 128
     finally:
         b.close()
+
+
+def test_headline_shape_launch_at_full_size():
+    # The headline workload's own launch: every lane of the default grid (2^24
+    # lanes on MI355X), 8 batches of 64 keys each, 2^33 keys in one launch.
+    # Every hit of the 7-hex prefix is re-derived on the CPU, every lane has
+    # walked exactly its 512 keys (sampled at both ends of the lane range and
+    # in the middle), and the launch counted every key.
+    start, _, ed_pub, prefix = seeded(1, 7)
+    b = lxv.Backend(batch=64)
+    try:
+        stride = lxv.lane_stride(b.threads)
+        b.init_seeded(start, stride)
+        hits, _ = b.launch(lxv.launch_args_for(ed_pub, prefix, iters=8, max_hits=1 << 12))
+        st = b.stats()
+        print("lanes", b.threads, "kernel", b.walk_kernel(), "hits", len(hits), "ms %.1f" % st.kernel_ms, flush=True)
+        assert st.keys == b.threads * 512 and st.stopped == 0 and st.hits_total == len(hits)
+        assert 8 <= len(hits) <= 96  # 2^33 / 16^7 = 32 expected
+        for t, n, addr in hits:
+            assert addr == orc.lxmf_address(orc.x25519(orc.step_scalar(start, t * stride + n)), ed_pub)
+            assert addr.hex().startswith(prefix)
+        for first in (0, b.threads // 2, b.threads - 1000):
+            assert b.read_ns(first, 1000) == [512] * 1000
+    finally:
+        b.close()
