@@ -27,6 +27,7 @@ share of SIMD cycles issuing VALU work (a wave64 instruction occupies a
 import argparse
 import csv
 import ctypes
+import functools
 import json
 import os
 import re
@@ -57,6 +58,7 @@ def kernel_attrs(layout, batch, wpc, walk):
     return {"vgprs": v.value, "scratch_bytes": sc.value, "waves_per_simd": w.value}
 
 
+@functools.lru_cache(maxsize=None)
 def device_cus():
     b = lxv.Backend(threads=64)
     try:
