@@ -18,6 +18,8 @@ merged, VALU busy %. The fastest point is the pick; the table goes to JSON.
     rocprofv3 --pmc SQ_ACTIVE_INST_VALU SQ_INSTS_VALU GRBM_GUI_ACTIVE --output-format csv \\
         -d OUT -o pmc -- python3 autotune.py --budget-log2 28 --out /dev/null
     python autotune.py --merge-pmc OUT/pmc_counter_collection.csv --out tuned.json
+    # separate sweeps (say one per walk form) joined, picked anew and merged
+    python autotune.py --join a.json b.json [--merge-pmc CSV] --out tuned.json
 
 VALU busy % = SQ_ACTIVE_INST_VALU x 4 / (SIMDs x GRBM_GUI_ACTIVE / XCDs): the
 share of SIMD cycles issuing VALU work (a wave64 instruction occupies a
@@ -188,11 +190,20 @@ def main():
     ap.add_argument("--budget-log2", type=int, default=34)
     ap.add_argument("--out", default=os.path.join(HERE, "tuned.json"))
     ap.add_argument("--merge-pmc", default=None, help="PMC csv of a profiled pass to merge into --out")
+    ap.add_argument("--join", nargs="+", default=None,
+                    help="tables of separate sweeps (e.g. one per walk form) to join into --out, picking anew")
     ap.add_argument("--walk", choices=("all", "pairs", "projective"), default="all")
     ap.add_argument("--shape", choices=("grid", "round"), default="grid",
                     help="grid: the library's launch shape at each point's occupancy; round: one resident round")
     args = ap.parse_args()
-    if args.merge_pmc:
+    if args.join:
+        tables = [load(p) for p in args.join]
+        if any(t is None for t in tables):
+            raise SystemExit("cannot read every table of %s" % args.join)
+        table = pick({"workload": tables[0]["workload"], "points": [p for t in tables for p in t["points"]]})
+        if args.merge_pmc:
+            table = merge_pmc(table, args.merge_pmc)
+    elif args.merge_pmc:
         table = load(args.out)
         if table is None:
             raise SystemExit("no table at %s to merge into" % args.out)
