@@ -106,6 +106,57 @@ def load_traffic():
         return None
 
 
+def pmc_summary(out_dir):
+    """profiles/walk_pmc.json from three PMC runs of this script (one launch
+    each, --copy-probe-mib 1024) under out_dir/pmc_sq, pmc_fetch and
+    pmc_write, with the first run's log as out_dir/pmc_sq.log: VALU
+    instructions and HBM bytes per key of the walk kernel the run launched,
+    FETCH_SIZE and WRITE_SIZE scaled by what they read for the 1 GiB copy."""
+    import csv
+    import glob
+
+    def find(pattern):
+        g = glob.glob(os.path.join(out_dir, pattern), recursive=True)
+        if not g:
+            raise SystemExit("missing %s under %s" % (pattern, out_dir))
+        return g[0]
+
+    line = [ln for ln in open(os.path.join(out_dir, "pmc_sq.log")) if ln.startswith('{"metric"')][-1]
+    run = json.loads(line)
+    kernel, keys = run["config"]["kernel"], run["keys"]
+
+    def counters(path):
+        walk, other = {}, {}
+        for r in csv.DictReader(open(path)):
+            tgt = walk if kernel in r["Kernel_Name"] else other.setdefault(r["Kernel_Name"], {})
+            tgt[r["Counter_Name"]] = tgt.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        return walk, other
+
+    sq, _ = counters(find("pmc_sq/**/*_counter_collection.csv"))
+    fe, fo = counters(find("pmc_fetch/**/*_counter_collection.csv"))
+    wr, wo = counters(find("pmc_write/**/*_counter_collection.csv"))
+    copy = [k for k in fo if "copy_kernel" in k][0]
+    out = {"kernel": kernel, "keys_per_launch": keys,
+           "valu_insts_per_key": sq["SQ_INSTS_VALU"] * 64 / keys,
+           "salu_insts_per_key": sq["SQ_INSTS_SALU"] * 64 / keys,
+           "vmem_rd_insts_per_key": sq["SQ_INSTS_VMEM_RD"] * 64 / keys,
+           "vmem_wr_insts_per_key": sq["SQ_INSTS_VMEM_WR"] * 64 / keys,
+           "valu_busy_pct": 100.0 * sq["SQ_ACTIVE_INST_VALU"] * 4 / (1024 * sq["GRBM_GUI_ACTIVE"] / 8),
+           "fetch_size_kib": fe["FETCH_SIZE"], "write_size_kib": wr["WRITE_SIZE"],
+           "copy_probe_bytes": 1 << 30, "copy_fetch_kib": fo[copy]["FETCH_SIZE"],
+           "copy_write_kib": wo[copy]["WRITE_SIZE"]}
+    out["fetch_scale"] = (1 << 30) / 1024 / out["copy_fetch_kib"]
+    out["write_scale"] = (1 << 30) / 1024 / out["copy_write_kib"]
+    out["hbm_read_bytes_per_key"] = out["fetch_size_kib"] * 1024 * out["fetch_scale"] / keys
+    out["hbm_write_bytes_per_key"] = out["write_size_kib"] * 1024 * out["write_scale"] / keys
+    out["hbm_bytes_per_key"] = out["hbm_read_bytes_per_key"] + out["hbm_write_bytes_per_key"]
+    out["note"] = ("FETCH_SIZE/WRITE_SIZE of the walk kernel scaled by the factor that makes the same counters read "
+                   "a 1 GiB plain copy as 1 GiB; per-key figure x keys per launch = traffic per launch")
+    with open(os.path.join(ROOT, "profiles", "walk_pmc.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
+
+
 def plain_rate(batch=256, waves_per_cu=16, budget_log2=33):
     """keys/s of the plain kernel on one search (workload seed 2, 8 hex
     digits), kernel time only: the yardstick for the many-search workloads."""
@@ -204,7 +255,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one plain copy kernel of this many MiB (counter calibration)")
+    ap.add_argument("--pmc-summary", metavar="DIR", default=None,
+                    help="write profiles/walk_pmc.json from the three PMC runs under DIR, then exit (no GPU)")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the library walks on a single MI355X and has no multi-GPU path "
               "(a search would shard by disjoint lane ranges with no exchange).", file=sys.stderr)
