@@ -106,7 +106,7 @@ def load_traffic():
         return None
 
 
-def pmc_summary(out_dir):
+def pmc_summary(out_dir, dest=os.path.join(ROOT, "profiles", "walk_pmc.json")):
     """profiles/walk_pmc.json from three PMC runs of this script (one launch
     each, --copy-probe-mib 1024) under out_dir/pmc_sq, pmc_fetch and
     pmc_write, with the first run's log as out_dir/pmc_sq.log: VALU
@@ -152,9 +152,10 @@ def pmc_summary(out_dir):
     out["hbm_bytes_per_key"] = out["hbm_read_bytes_per_key"] + out["hbm_write_bytes_per_key"]
     out["note"] = ("FETCH_SIZE/WRITE_SIZE of the walk kernel scaled by the factor that makes the same counters read "
                    "a 1 GiB plain copy as 1 GiB; per-key figure x keys per launch = traffic per launch")
-    with open(os.path.join(ROOT, "profiles", "walk_pmc.json"), "w") as f:
+    with open(dest, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out, indent=1))
+    return out
 
 
 def plain_rate(batch=256, waves_per_cu=16, budget_log2=33):
