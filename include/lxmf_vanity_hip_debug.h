@@ -24,10 +24,17 @@ extern "C" {
 #define LXV_FE_CANON 5
 #define LXV_FE_MUL121665 6
 #define LXV_FE_INVERT_CHAIN 7  /* z^(p-2) by the reference's addition chain (LXV_FE_INVERT: divsteps) */
+/* Or-ed into an op (LXV_LAYOUT_8X32 only): inputs are raw limbs (any value
+ * below 2^256, bit 255 kept), and the output is the limbs the operation
+ * produced (congruent to the result mod p, not canonical), except for
+ * LXV_FE_CANON, which still returns the canonical encoding. Sums and
+ * differences are returned as computed, not multiplied by one. */
+#define LXV_FE_RAW_LIMBS 0x100
 
 /* out[i] = canonical encoding of op(a[i], b[i]) in limb layout `layout`
  * (LXV_LAYOUT_*), for n elements of 8 little-endian words each (bit 255 of
- * the inputs dropped). on_gpu selects the device kernel (device 0) or the
+ * the inputs dropped; see LXV_FE_RAW_LIMBS for the 8 x 32 layout's own
+ * representation). on_gpu selects the device kernel (device 0) or the
  * host build of the same code. Returns LXV_OK or a negative status. */
 int lxv_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n, int on_gpu);
 
@@ -42,6 +49,30 @@ int lxv_debug_address(const uint32_t ed[8], const uint32_t nh[3], const uint32_t
  * peak the walk's multiply count is priced against. Also returns the
  * device's clock and CU count. */
 int lxv_debug_mad_rate(int device, double* mads_per_s, double* clock_hz, int* cus);
+
+/* VALU issue-rate probes for lxv_debug_valu_rate: one instruction in 8
+ * independent chains per lane, or the SHA-256 round mix. */
+#define LXV_VALU_ALIGNBIT 0     /* v_alignbit_b32 (rotates) */
+#define LXV_VALU_BITOP3 1       /* v_bitop3_b32 (XOR3, Ch, Maj) */
+#define LXV_VALU_ADD3 2         /* v_add3_u32 */
+#define LXV_VALU_ADD 3          /* v_add_u32 */
+#define LXV_VALU_XOR 4          /* v_xor_b32 */
+#define LXV_VALU_PERM 5         /* v_perm_b32 (byte swaps) */
+#define LXV_VALU_SHIFT 6        /* v_lshrrev_b32 */
+#define LXV_VALU_MAD64 7        /* v_mad_u64_u32 */
+#define LXV_VALU_SHA_MIX 8      /* 3 v_alignbit : 2 v_bitop3 : 2 v_add3, as a SHA-256 round */
+#define LXV_VALU_ADD_CO 9       /* v_add_co_u32 (carry out to VCC) */
+#define LXV_VALU_CARRY_CHAIN 10 /* v_addc_co_u32 chains through VCC, each reading the last's carry */
+#define LXV_VALU_WALK_MIX 11    /* the instruction classes of the pair walk's per-key loop, in its proportions */
+
+/* Measures the issue rate of probe `op` on `device` with one resident round
+ * of waves_per_cu waves per CU (a multiple of 4, at most 32): lane-level
+ * instructions per second (wave instructions x 64). clock_hz is the shader
+ * clock the probe's waves saw (s_memtime against the constant-rate
+ * s_memrealtime over each block's run), nominal_clock_hz the device's
+ * reported peak clock. */
+int lxv_debug_valu_rate(int device, int op, uint32_t waves_per_cu, double* lane_ops_per_s, double* clock_hz,
+                        double* nominal_clock_hz, int* cus);
 
 /* Runs one plain device-to-device copy kernel of `bytes` (a multiple of 16)
  * on `device` and returns its time: the known-size reference that HBM byte

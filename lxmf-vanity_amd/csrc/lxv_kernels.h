@@ -13,24 +13,44 @@ struct WalkParams;
 enum DebugFeOp {
     kFeMul = 0, kFeSq = 1, kFeAdd = 2, kFeSub = 3, kFeInvert = 4, kFeCanon = 5, kFeMulSmall = 6, kFeInvertChain = 7
 };
+// Or-ed into an op code (8 x 32 layout only): the inputs are taken as raw
+// limbs, every value below 2^256 including those at or above 2^255, and the
+// result is returned as the limbs the operation produced (weakly reduced,
+// congruent to the result) instead of its canonical encoding; kFeCanon still
+// returns the canonical encoding of its raw input.
+constexpr int kFeRawLimbs = 0x100;
 
 // out = canonical words of op(a, b); a and b are decoded as 8 little-endian
 // words (bit 255 dropped). Sums and differences are multiplied by one before
-// encoding, as the walk only encodes multiplication results.
+// encoding, as the walk only encodes multiplication results. With
+// kFeRawLimbs (F = Field8): raw limbs in and out, see above.
 template <class F>
 LXV_HD void debug_fe_op(int op, const uint32_t a[8], const uint32_t b[8], uint32_t out[8]) {
     typedef typename F::fe fe;
-    const fe x = F::from_words(a), y = F::from_words(b);
+    const bool raw = (op & kFeRawLimbs) != 0 && F::kLimbs == 8;
+    op &= 0xff;
+    uint32_t la[F::kLimbs] = {}, lb[F::kLimbs] = {};
+    for (int i = 0; i < 8 && i < F::kLimbs; i++) {
+        la[i] = a[i];
+        lb[i] = b[i];
+    }
+    const fe x = raw ? F::load(la) : F::from_words(a), y = raw ? F::load(lb) : F::from_words(b);
     fe r;
     switch (op) {
         case kFeMul: r = F::mul(x, y); break;
         case kFeSq: r = F::sq(x); break;
-        case kFeAdd: r = F::mul(F::add(x, y), F::one()); break;
-        case kFeSub: r = F::mul(F::sub(x, y), F::one()); break;
+        case kFeAdd: r = raw ? F::add(x, y) : F::mul(F::add(x, y), F::one()); break;
+        case kFeSub: r = raw ? F::sub(x, y) : F::mul(F::sub(x, y), F::one()); break;
         case kFeInvert: r = F::invert(x); break;
         case kFeInvertChain: r = F::invert_chain(x); break;
         case kFeMulSmall: r = F::mul_small(x, 121665); break;
-        default: r = F::mul(x, F::one()); break;
+        default: r = raw ? x : F::mul(x, F::one()); break;
+    }
+    if (raw && op != kFeCanon) {
+        uint32_t lo[F::kLimbs];
+        F::store(r, lo);
+        for (int i = 0; i < 8 && i < F::kLimbs; i++) out[i] = lo[i];
+        return;
     }
     F::to_words(r, out);
 }
@@ -101,6 +121,14 @@ hipError_t launch_debug_fe(int layout, int op, const uint32_t* a, const uint32_t
                            hipStream_t s);
 // lanes must be a multiple of 256; 8 mads per lane per iteration.
 hipError_t launch_mad_rate(uint32_t* out, uint32_t lanes, uint32_t seed, uint32_t iters, hipStream_t s);
+// VALU issue-rate probe `op` (LXV_VALU_* of lxmf_vanity_hip_debug.h) over
+// `blocks` blocks of 256 lanes: out[lane] (blocks x 256 words) and, per block,
+// 4 clock readings (shader clock at start and end, constant-rate clock at
+// start and end) in clocks[4 block ..]. valu_probe_ops_per_iter: the probe's
+// VALU instructions per lane per iteration (0: no such probe).
+int valu_probe_ops_per_iter(int op);
+hipError_t launch_valu_rate(int op, uint32_t* out, unsigned long long* clocks, uint32_t blocks, uint32_t seed,
+                            uint32_t iters, hipStream_t s);
 // Copies bytes (a multiple of 16) from src to dst.
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_debug_addr(const ShaSearch& c, const uint32_t* u, uint32_t* out, uint32_t n, hipStream_t s);

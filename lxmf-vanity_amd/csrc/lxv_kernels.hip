@@ -119,6 +119,157 @@ __global__ __launch_bounds__(256) void mad_rate_kernel(uint32_t* out, uint32_t s
     out[lane] = (uint32_t)(w0 ^ w1 ^ w2 ^ w3 ^ w4 ^ w5 ^ w6 ^ (w7 >> 3));
 }
 
+// Issue-rate probes of single VALU instructions and of the SHA-256 mix, at
+// a chosen occupancy: 8 independent chains per lane (so a wave never waits on
+// its own results), each loop iteration kValuOpsPerIter[op] instructions,
+// exactly as written (register operands only; iters a multiple of 4, the
+// loop's unroll). Thread 0 of every block
+// records the shader clock (s_memtime) and the constant-rate clock
+// (s_memrealtime) at its start and end, so the host can tell the clock the
+// SIMDs actually ran at from the kernel itself.
+enum ValuProbeOp {
+    kVpAlignbit = 0, kVpBitop3 = 1, kVpAdd3 = 2, kVpAdd = 3, kVpXor = 4, kVpPerm = 5, kVpShift = 6,
+    kVpMad64 = 7, kVpShaMix = 8, kVpAddCo = 9, kVpCarryChain = 10, kVpWalkMix = 11, kVpOps = 12
+};
+constexpr int kValuOpsPerIter[kVpOps] = {8, 8, 8, 8, 8, 8, 8, 8, 56, 8, 9, 200};
+
+#define LXV_R8(I) I(0) I(1) I(2) I(3) I(4) I(5) I(6) I(7)
+
+template <int kOp>
+__global__ __launch_bounds__(256) void valu_rate_kernel(uint32_t* out, unsigned long long* clocks, uint32_t seed,
+                                                        uint32_t iters) {
+    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long c0 = 0, r0 = 0;
+    if (threadIdx.x == 0) {
+        c0 = clock64();
+        r0 = wall_clock64();
+    }
+    uint32_t x0 = seed ^ lane, x1 = x0 * 3u, x2 = x0 * 5u, x3 = x0 * 7u, x4 = x0 * 9u, x5 = x0 * 11u,
+             x6 = x0 * 13u, x7 = x0 * 15u;
+    uint64_t w0 = x0, w1 = x1, w2 = x2, w3 = x3, w4 = x4, w5 = x5, w6 = x6, w7 = x7;
+    const uint32_t y = seed + 0x9e3779b9u, z = seed * 0x85ebca6bu;
+#pragma unroll 4
+    for (uint32_t it = 0; it < iters; it++) {
+        if constexpr (kOp == kVpAlignbit) {
+#define LXV_I(k) "v_alignbit_b32 %" #k ", %" #k ", %8, 7\n"
+            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+                         "+v"(x7) : "v"(y), "v"(z));
+#undef LXV_I
+        } else if constexpr (kOp == kVpBitop3) {
+#define LXV_I(k) "v_bitop3_b32 %" #k ", %" #k ", %8, %9 bitop3:0x96\n"
+            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+                         "+v"(x7) : "v"(y), "v"(z));
+#undef LXV_I
+        } else if constexpr (kOp == kVpAdd3) {
+#define LXV_I(k) "v_add3_u32 %" #k ", %" #k ", %8, %9\n"
+            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+                         "+v"(x7) : "v"(y), "v"(z));
+#undef LXV_I
+        } else if constexpr (kOp == kVpAdd) {
+#define LXV_I(k) "v_add_u32_e32 %" #k ", %8, %" #k "\n"
+            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+                         "+v"(x7) : "v"(y), "v"(z));
+#undef LXV_I
+        } else if constexpr (kOp == kVpXor) {
+#define LXV_I(k) "v_xor_b32_e32 %" #k ", %8, %" #k "\n"
+            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+                         "+v"(x7) : "v"(y), "v"(z));
+#undef LXV_I
+        } else if constexpr (kOp == kVpPerm) {
+#define LXV_I(k) "v_perm_b32 %" #k ", %" #k ", %8, %9\n"
+            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+                         "+v"(x7) : "v"(y), "v"(z));
+#undef LXV_I
+        } else if constexpr (kOp == kVpShift) {
+#define LXV_I(k) "v_lshrrev_b32_e32 %" #k ", %8, %" #k "\n"
+            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+                         "+v"(x7) : "v"(y), "v"(z));
+#undef LXV_I
+        } else if constexpr (kOp == kVpMad64) {
+            uint64_t c0_, c1_, c2_, c3_, c4_, c5_, c6_, c7_;
+            asm volatile(
+                "v_mad_u64_u32 %0, %8, %16, %17, %0\n v_mad_u64_u32 %1, %9, %16, %17, %1\n"
+                "v_mad_u64_u32 %2, %10, %16, %17, %2\n v_mad_u64_u32 %3, %11, %16, %17, %3\n"
+                "v_mad_u64_u32 %4, %12, %16, %17, %4\n v_mad_u64_u32 %5, %13, %16, %17, %5\n"
+                "v_mad_u64_u32 %6, %14, %16, %17, %6\n v_mad_u64_u32 %7, %15, %16, %17, %7"
+                : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5), "+v"(w6), "+v"(w7), "=s"(c0_),
+                  "=s"(c1_), "=s"(c2_), "=s"(c3_), "=s"(c4_), "=s"(c5_), "=s"(c6_), "=s"(c7_)
+                : "v"(y), "v"(z));
+        } else if constexpr (kOp == kVpShaMix) {
+            // per chain: 3 rotates, 2 three-input boolean functions, 2
+            // three-input adds (the proportions of a SHA-256 round's
+            // Sigma + Ch/Maj + sums)
+#define LXV_I(k) "v_alignbit_b32 %" #k ", %" #k ", %" #k ", 6\n"
+#define LXV_J(k) "v_alignbit_b32 %" #k ", %" #k ", %8, 11\n"
+#define LXV_K(k) "v_bitop3_b32 %" #k ", %" #k ", %8, %9 bitop3:0x96\n"
+#define LXV_L(k) "v_add3_u32 %" #k ", %" #k ", %8, %9\n"
+#define LXV_M(k) "v_bitop3_b32 %" #k ", %" #k ", %9, %8 bitop3:0xca\n"
+            asm volatile(LXV_R8(LXV_I) LXV_R8(LXV_J) LXV_R8(LXV_K) LXV_R8(LXV_L) LXV_R8(LXV_I) LXV_R8(LXV_M)
+                             LXV_R8(LXV_L)
+                         : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6), "+v"(x7)
+                         : "v"(y), "v"(z));
+#undef LXV_I
+#undef LXV_J
+#undef LXV_K
+#undef LXV_L
+#undef LXV_M
+        } else if constexpr (kOp == kVpAddCo) {
+#define LXV_I(k) "v_add_co_u32_e32 %" #k ", vcc, %8, %" #k "\n"
+            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+                         "+v"(x7) : "v"(y), "v"(z) : "vcc");
+#undef LXV_I
+        } else if constexpr (kOp == kVpWalkMix) {
+            // the per-key instruction classes of the pair walk's loop (its
+            // gfx950 assembly at B = 64, 4 waves per SIMD: 30% v_alignbit, 19%
+            // v_bitop3, 12% v_add3, 9% v_addc_co_u32_e64, 8% v_mad_u64_u32, 8%
+            // v_add_u32, 6% v_mov, 3% v_lshrrev), 25 per chain per iteration
+            uint64_t cout_;
+#define LXV_A(k) "v_alignbit_b32 %" #k ", %" #k ", %" #k ", 6\n"
+#define LXV_B(k) "v_bitop3_b32 %" #k ", %" #k ", %17, %18 bitop3:0x96\n"
+#define LXV_C(k) "v_add3_u32 %" #k ", %" #k ", %17, %18\n"
+#define LXV_D(k) "v_addc_co_u32_e64 %" #k ", %16, %" #k ", %17, %19\n"
+#define LXV_E(k) "v_mad_u64_u32 %[w" #k "], %16, %17, %18, %[w" #k "]\n"
+#define LXV_F(k) "v_add_u32_e32 %" #k ", %17, %" #k "\n"
+#define LXV_G(k) "v_mov_b32_e32 %" #k ", %18\n"
+#define LXV_H(k) "v_lshrrev_b32_e32 %" #k ", %17, %" #k "\n"
+            asm volatile(LXV_R8(LXV_A) LXV_R8(LXV_A) LXV_R8(LXV_A) LXV_R8(LXV_B) LXV_R8(LXV_B) LXV_R8(LXV_C)
+                         LXV_R8(LXV_A) LXV_R8(LXV_A) LXV_R8(LXV_A) LXV_R8(LXV_B) LXV_R8(LXV_C) LXV_R8(LXV_D)
+                         LXV_R8(LXV_E) LXV_R8(LXV_F) LXV_R8(LXV_G) LXV_R8(LXV_A) LXV_R8(LXV_A) LXV_R8(LXV_B)
+                         LXV_R8(LXV_B) LXV_R8(LXV_C) LXV_R8(LXV_D) LXV_R8(LXV_E) LXV_R8(LXV_F) LXV_R8(LXV_G)
+                         LXV_R8(LXV_H)
+                         : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6), "+v"(x7),
+                           [w0] "+v"(w0), [w1] "+v"(w1), [w2] "+v"(w2), [w3] "+v"(w3), [w4] "+v"(w4), [w5] "+v"(w5),
+                           [w6] "+v"(w6), [w7] "+v"(w7), "=&s"(cout_)
+                         : "v"(y), "v"(z), "s"((uint64_t)0));
+#undef LXV_A
+#undef LXV_B
+#undef LXV_C
+#undef LXV_D
+#undef LXV_E
+#undef LXV_F
+#undef LXV_G
+#undef LXV_H
+        } else {
+            // one dependent carry chain through VCC (the shape of a field
+            // addition): each instruction reads the carry the previous wrote
+#define LXV_I(k) "v_addc_co_u32_e32 %" #k ", vcc, %8, %" #k ", vcc\n"
+            asm volatile("v_add_co_u32_e32 %0, vcc, %8, %0\n" LXV_R8(LXV_I)
+                         : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6), "+v"(x7)
+                         : "v"(y), "v"(z) : "vcc");
+#undef LXV_I
+        }
+    }
+    out[lane] = x0 ^ x1 ^ x2 ^ x3 ^ x4 ^ x5 ^ x6 ^ x7 ^ (uint32_t)(w0 ^ w1 ^ w2 ^ w3 ^ w4 ^ w5 ^ w6 ^ (w7 >> 3));
+    if (threadIdx.x == 0) {
+        unsigned long long* c = clocks + 4ull * blockIdx.x;
+        c[0] = c0;
+        c[1] = clock64();
+        c[2] = r0;
+        c[3] = wall_clock64();
+    }
+}
+#undef LXV_R8
+
 // A plain copy of known size, for scaling HBM byte counters.
 __global__ __launch_bounds__(256) void copy_kernel(const uint4* src, uint4* dst, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -377,6 +528,20 @@ hipError_t launch_debug_fe(int layout, int op, const uint32_t* a, const uint32_t
 
 hipError_t launch_mad_rate(uint32_t* out, uint32_t lanes, uint32_t seed, uint32_t iters, hipStream_t s) {
     hipLaunchKernelGGL(mad_rate_kernel, dim3(lanes / 256), dim3(256), 0, s, out, seed, iters);
+    return hipGetLastError();
+}
+
+int valu_probe_ops_per_iter(int op) { return op >= 0 && op < kVpOps ? kValuOpsPerIter[op] : 0; }
+
+hipError_t launch_valu_rate(int op, uint32_t* out, unsigned long long* clocks, uint32_t blocks, uint32_t seed,
+                            uint32_t iters, hipStream_t s) {
+    typedef void (*probe_fn)(uint32_t*, unsigned long long*, uint32_t, uint32_t);
+    static const probe_fn fns[kVpOps] = {
+        valu_rate_kernel<0>, valu_rate_kernel<1>, valu_rate_kernel<2>, valu_rate_kernel<3>,
+        valu_rate_kernel<4>, valu_rate_kernel<5>, valu_rate_kernel<6>, valu_rate_kernel<7>,
+        valu_rate_kernel<8>, valu_rate_kernel<9>, valu_rate_kernel<10>, valu_rate_kernel<11>};
+    if (op < 0 || op >= kVpOps) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fns[op], dim3(blocks), dim3(256), 0, s, out, clocks, seed, iters);
     return hipGetLastError();
 }
 

@@ -540,8 +540,9 @@ extern "C" void lxv_seeded_search(uint64_t seed, int nibbles, uint8_t start[32],
 
 extern "C" int lxv_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n,
                             int on_gpu) {
-    if (!a || !b || !out || op < 0 || op > 7) return LXV_E_ARG;
+    if (!a || !b || !out || (op & 0xff) > 7 || (op & ~(0xff | LXV_FE_RAW_LIMBS)) != 0) return LXV_E_ARG;
     if (layout != LXV_LAYOUT_10X26 && layout != LXV_LAYOUT_8X32) return LXV_E_ARG;
+    if ((op & LXV_FE_RAW_LIMBS) && layout != LXV_LAYOUT_8X32) return LXV_E_ARG;
     if (!on_gpu) {
         for (uint32_t i = 0; i < n; i++) {
             if (layout == LXV_LAYOUT_8X32)
@@ -625,6 +626,56 @@ extern "C" int lxv_debug_mad_rate(int device, double* mads_per_s, double* clock_
     if (e != hipSuccess || ms <= 0) return LXV_E_HIP;
     *mads_per_s = 5.0 * lanes * (double)iters * 8.0 / (ms * 1e-3);
     if (clock_hz) *clock_hz = prop.clockRate * 1e3;
+    if (cus) *cus = prop.multiProcessorCount;
+    return LXV_OK;
+}
+
+extern "C" int lxv_debug_valu_rate(int device, int op, uint32_t waves_per_cu, double* lane_ops_per_s,
+                                   double* clock_hz, double* nominal_clock_hz, int* cus) {
+    const int per_iter = valu_probe_ops_per_iter(op);
+    if (!lane_ops_per_s || per_iter == 0 || waves_per_cu == 0 || waves_per_cu % 4 || waves_per_cu > 32)
+        return LXV_E_ARG;
+    hipDeviceProp_t prop;
+    int wall_khz = 0;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+        hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess)
+        return LXV_E_HIP;
+    // one resident round at the asked occupancy: blocks of 4 waves
+    const uint32_t blocks = (uint32_t)prop.multiProcessorCount * (waves_per_cu / 4);
+    const uint32_t lanes = blocks * 256u, iters = 16384, runs = 5;
+    uint32_t* out = nullptr;
+    unsigned long long* clocks = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc((void**)&out, (size_t)lanes * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&clocks, (size_t)blocks * runs * 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_valu_rate(op, out, clocks, blocks, 1, iters / 8, nullptr);  // warm-up
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    for (uint32_t r = 0; r < runs && e == hipSuccess; r++)
+        e = launch_valu_rate(op, out, clocks + (size_t)r * blocks * 4, blocks, 2 + r, iters, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    std::vector<unsigned long long> h((size_t)blocks * runs * 4);
+    if (e == hipSuccess) e = hipMemcpy(h.data(), clocks, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    (void)hipFree(clocks);
+    if (e != hipSuccess || ms <= 0) return LXV_E_HIP;
+    *lane_ops_per_s = (double)runs * lanes * (double)iters * per_iter / (ms * 1e-3);
+    // the shader clock each block saw over its own run, against the
+    // constant-rate clock, averaged over blocks
+    double cyc = 0, real = 0;
+    for (size_t b = 0; b < (size_t)blocks * runs; b++) {
+        cyc += (double)(h[4 * b + 1] - h[4 * b]);
+        real += (double)(h[4 * b + 3] - h[4 * b + 2]);
+    }
+    if (clock_hz) *clock_hz = real > 0 ? cyc / real * wall_khz * 1e3 : 0;
+    if (nominal_clock_hz) *nominal_clock_hz = prop.clockRate * 1e3;
     if (cus) *cus = prop.multiProcessorCount;
     return LXV_OK;
 }

@@ -29,6 +29,7 @@ LXV_OK = 0
 LXV_E_BUSY = -5
 LAYOUT_8X32 = 0
 LAYOUT_10X26 = 1
+FE_RAW_LIMBS = 0x100            # lxv_debug_fe: raw 8 x 32 limbs in and out (bit 255 kept)
 WALK_PAIRS = 0                  # batch keys around an affine centre (the default)
 WALK_PROJECTIVE = 1             # the reference's chained x-only differential additions
 MAX_HITS = 4096                 # hits recorded per launch (src/gpu/mod.rs:23)
@@ -474,6 +475,30 @@ def debug_fe(op, a_words, b_words, on_gpu, layout=LAYOUT_8X32):
     if rc != LXV_OK:
         raise RuntimeError("lxv_debug_fe failed (%d)" % rc)
     return list(O[:8 * n])
+
+
+def debug_valu_rate(op, waves_per_cu=16, device=0):
+    """VALU issue-rate probe `op` (LXV_VALU_* of the debug header) at
+    waves_per_cu resident waves per CU: (lane-level instructions per second,
+    the shader clock the probe measured in Hz, the device's nominal clock in
+    Hz, CUs)."""
+    L = lib()
+    d = ctypes.POINTER(ctypes.c_double)
+    L.lxv_debug_valu_rate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, d, d, d,
+                                      ctypes.POINTER(ctypes.c_int)]
+    rate, clk, nom, cus = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int(0)
+    rc = L.lxv_debug_valu_rate(device, op, waves_per_cu, ctypes.byref(rate), ctypes.byref(clk), ctypes.byref(nom),
+                               ctypes.byref(cus))
+    if rc != LXV_OK:
+        raise RuntimeError("lxv_debug_valu_rate failed (%d)" % rc)
+    return rate.value, clk.value, nom.value, cus.value
+
+
+VALU_PROBES = {0: "v_alignbit_b32", 1: "v_bitop3_b32", 2: "v_add3_u32", 3: "v_add_u32", 4: "v_xor_b32",
+               5: "v_perm_b32", 6: "v_lshrrev_b32", 7: "v_mad_u64_u32",
+               8: "SHA-256 round mix (3 alignbit : 2 bitop3 : 2 add3)", 9: "v_add_co_u32",
+               10: "v_addc_co_u32 dependent carry chain",
+               11: "pair-walk mix (30% alignbit, 19% bitop3, 12% add3, 9% addc, 8% mad64, 8% add, 6% mov, 3% shr)"}
 
 
 def debug_address(ed_words, nh_words, u_words, on_gpu):

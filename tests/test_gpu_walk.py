@@ -57,6 +57,27 @@ def test_gpu_field_ops_match_big_integers(layout):
         assert got == want, op
 
 
+@pytest.mark.parametrize("op", [0, 1, 2, 3, 5, 6, 4, 7])
+def test_gpu_fe8_raw_limbs_weakly_reduced_inputs(op):
+    # The 8 x 32 asm on its own working range: raw limbs in [0, 2^256) with
+    # bit 255 kept (values near 2^256, all-ones words, sums that wrap twice,
+    # differences that borrow twice), results congruent mod p; the canonical
+    # encoding (op 5) of every such value is exact.
+    import fe_edge
+    pairs = fe_edge.cases()
+    if op in (4, 7):
+        pairs = pairs[::5]
+    A = sum((fe_edge.words(a) for a, _ in pairs), [])
+    B = sum((fe_edge.words(b) for _, b in pairs), [])
+    print("raw op", op, "cases", len(pairs), flush=True)
+    out = lxv.debug_fe(op | lxv.FE_RAW_LIMBS, A, B, on_gpu=True, layout=lxv.LAYOUT_8X32)
+    assert fe_edge.check(op, pairs, out, raw=True) == []
+    # the same limbs through the host build give the same residues
+    host = lxv.debug_fe(op | lxv.FE_RAW_LIMBS, A, B, on_gpu=False, layout=lxv.LAYOUT_8X32)
+    assert [fe_edge.value(out[8 * i:8 * i + 8]) % P for i in range(len(pairs))] == \
+        [fe_edge.value(host[8 * i:8 * i + 8]) % P for i in range(len(pairs))]
+
+
 def test_gpu_address_hash_matches_oracle():
     rng = random.Random(32)
     ed_pub = bytes(rng.getrandbits(8) for _ in range(32))
@@ -225,6 +246,52 @@ def test_pair_walk_every_batch_size(layout, batch):
         b.close()
 
 
+# Every occupancy the library builds: waves_per_cu 4, 8 and 16 select the
+# kernels compiled for 1, 2 and 4 waves per SIMD (different register budgets
+# and spills), in both layouts, both walk forms and at B = 16, 64 and 256;
+# the pair walk also on its persistent grid.
+OCCUPANCY = [(walk, layout, batch, wpc, queue)
+             for walk in (lxv.WALK_PAIRS, lxv.WALK_PROJECTIVE)
+             for layout in (lxv.LAYOUT_8X32, lxv.LAYOUT_10X26)
+             for batch in (16, 64, 256)
+             for wpc in (4, 8, 16)
+             for queue in ((False, True) if walk == lxv.WALK_PAIRS else (False,))]
+
+
+@pytest.mark.parametrize("walk,layout,batch,wpc,queue", OCCUPANCY,
+                         ids=["%s-%s-b%d-w%d%s" % ("pairs" if w == lxv.WALK_PAIRS else "proj",
+                                                   "8x32" if l == lxv.LAYOUT_8X32 else "10x26", b, c,
+                                                   "-queue" if q else "")
+                              for w, l, b, c, q in OCCUPANCY])
+def test_walk_at_every_occupancy(walk, layout, batch, wpc, queue):
+    lanes = 512
+    b = lxv.Backend(threads=lanes, batch=batch, layout=layout, walk=walk, waves_per_cu=wpc,
+                    resident_lanes=128 if queue else 0)
+    try:
+        W = {4: 1, 8: 2, 16: 4}[wpc]
+        name, grid = b.walk_kernel()
+        assert name.endswith(", %d, %d>" % (batch, W)), name
+        assert grid == (128 if queue else lanes)
+        start, _, ed_pub, prefix = seeded(900 + batch + wpc, 2)
+        b.init_seeded(start, STRIDE)
+        iters = max(1, 512 // batch)
+        hits, dump = b.launch(lxv.launch_args_for(ed_pub, prefix, iters=iters, dump_threads=3, max_hits=1 << 14))
+        for t in range(3):
+            for i in range(batch):
+                k = orc.step_scalar(start, t * STRIDE + i)
+                assert lxv.words_to_bytes_le(dump[(t * batch + i) * 8:(t * batch + i) * 8 + 8]) == orc.x25519(k), \
+                    (t, i)
+        per_lane = iters * batch
+        want, xor = orc.scan_ranges(start, ed_pub, [(t * STRIDE, per_lane) for t in range(lanes)], prefix,
+                                    cap=1 << 14)
+        assert len(want) > 100
+        assert sorted((t * STRIDE + n, a) for t, n, a in hits) == want
+        assert b.checksum() == xor
+        assert b.read_ns() == [per_lane] * lanes
+    finally:
+        b.close()
+
+
 def test_full_address_prefix_finds_exactly_that_key(small_backend):
     b = small_backend
     start, _, ed_pub, _ = seeded(49, 0)
@@ -320,6 +387,19 @@ def test_config1_subrange_checksum_matches_oracle():
         assert gpu_xor == xor
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("op", range(12))
+def test_valu_probe_rates_are_sane(op):
+    # The issue-rate probes the VALU roofline is priced against: each runs,
+    # reports a positive rate no more than twice one wave64 instruction per
+    # SIMD per 4 clocks at the clock it measured, and a clock within the
+    # range the device can run at.
+    rate, clk, nominal, cus = lxv.debug_valu_rate(op, 16)
+    print("probe", op, "%.3e lane-ops/s at %.3f GHz (nominal %.3f GHz), %d CUs" % (rate, clk / 1e9, nominal / 1e9,
+                                                                                  cus), flush=True)
+    assert cus > 0 and 0.3e9 < clk < 1.2 * nominal
+    assert 0 < rate < 2.0 * cus * 64 * clk
 
 
 def test_self_test_and_run():

@@ -178,6 +178,20 @@ def test_device_field_code_chains_on_host(layout):
         x, y = z, (z + y) % P
 
 
+@pytest.mark.parametrize("op", [0, 1, 2, 3, 5, 6, 4])
+def test_fe8_raw_limbs_on_host(op):
+    # The host build of the 8 x 32 field code on its own working range:
+    # inputs anywhere below 2^256 (bit 255 kept), outputs congruent mod p.
+    import fe_edge
+    pairs = fe_edge.cases(n_random=60 if op == 4 else 300)
+    if op == 4:
+        pairs = pairs[::7]
+    A = sum((fe_edge.words(a) for a, _ in pairs), [])
+    B = sum((fe_edge.words(b) for _, b in pairs), [])
+    out = lxv.debug_fe(op | lxv.FE_RAW_LIMBS, A, B, on_gpu=False, layout=lxv.LAYOUT_8X32)
+    assert fe_edge.check(op, pairs, out, raw=True) == []
+
+
 def test_device_address_hash_on_host():
     rng = random.Random(25)
     ed_pub = bytes(rng.getrandbits(8) for _ in range(32))
