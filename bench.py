@@ -70,18 +70,41 @@ def measure_mad_peak():
     return rate.value, clk.value, cus.value
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def host_cores():
+    """(threads to use, CPUs visible, why). One thread per core of the lease,
+    as the reference runs one search thread per core (available_parallelism,
+    src/main.rs:474-477). A GPU box leases a share of its host's CPUs per GPU
+    and says so in OMP_NUM_THREADS; sched_getaffinity may show the whole
+    host there, so the lease's figure wins when it is smaller."""
+    visible = os.cpu_count() or 1
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    lease = os.environ.get("OMP_NUM_THREADS", "")
+    if lease.isdigit() and 0 < int(lease) < visible:
+        return int(lease), visible, "OMP_NUM_THREADS=%s: the host CPUs leased with this GPU" % lease
+    return visible, visible, "every CPU in this process's affinity mask"
+
+
 def cpu_baseline(seconds=12.0):
     """The oracle walk (the reference's CPU path restated in C, one thread per
     core as the reference runs, SHA-NI when present) on a bounded sample."""
     import oracle_lib as orc
     start, ed_seed, prefix = lxv.seeded_search(1, 7)
     ed_pub = orc.ed25519_pub(ed_seed)
-    cores = os.cpu_count() or 1
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        pass
-    cores = min(cores, 16)
+    cores, visible, why = host_cores()
     # calibrate with a short run, then size the sample to `seconds`
     n = 1 << 18
     t0 = time.perf_counter()
@@ -94,7 +117,36 @@ def cpu_baseline(seconds=12.0):
     return {"value": n / dt, "unit": "keys/s", "cores": cores, "kind": "port",
             "sample": "%d consecutive keys of the same seeded search (7-hex prefix), %.1f s, SHA-NI=%s"
                       % (n, dt, orc.shani()),
-            "per_core": n / dt / cores}
+            "per_core": n / dt / cores, "cpu_model": cpu_model(), "cpus_visible": visible, "cores_reason": why}
+
+
+def keys_text(k):
+    return "2^%d" % (k.bit_length() - 1) if k & (k - 1) == 0 else "%.4g" % k
+
+
+def valu_issue_roofline(keys_per_s, insts_per_key, kname, busy_pct):
+    """The walk's VALU issue rate (lane-level instructions per second: keys/s
+    x VALU instructions per key from the PMC summary) against measured issue
+    ceilings at the kernel's occupancy: the probe with the pair walk's own
+    instruction mix (the peak), the SHA-256 round mix, and the fastest single
+    instruction class measured; plus the shader clock the probes ran at."""
+    waves = int(kname.rstrip(">").split(",")[-1]) if kname.endswith(">") else 4
+    wpc = 4 * waves
+    achieved = keys_per_s * insts_per_key
+    walk_mix, clk, nominal, cus = lxv.debug_valu_rate(11, wpc)
+    sha_mix = lxv.debug_valu_rate(8, wpc)[0]
+    singles = {lxv.VALU_PROBES[op]: lxv.debug_valu_rate(op, wpc)[0] for op in (0, 1, 2, 3, 7)}
+    best_name = max(singles, key=singles.get)
+    per4 = cus * 64 * clk
+    return {"valu_insts_per_key": insts_per_key, "achieved": achieved / 1e12, "peak": walk_mix / 1e12,
+            "unit": "T lane-instr/s", "frac": achieved / walk_mix,
+            "peak_source": "issue-rate probe with the pair walk's per-key instruction mix at %d waves/CU" % wpc,
+            "busy_pct_measured": busy_pct, "cus": cus, "clock_hz_measured": clk, "clock_hz_nominal": nominal,
+            "achieved_per_simd_4clk": achieved / per4, "walk_mix_per_simd_4clk": walk_mix / per4,
+            "sha_mix_per_simd_4clk": sha_mix / per4, "frac_vs_sha_mix": achieved / sha_mix,
+            "fastest_single_class": best_name, "fastest_single_per_simd_4clk": singles[best_name] / per4,
+            "frac_vs_fastest_single": achieved / singles[best_name],
+            "frac_vs_nominal_clock": achieved / (cus * 64 * nominal)}
 
 
 def load_traffic():
@@ -279,7 +331,13 @@ def main():
     b = search.backend
     per_iter = b.threads * b.batch
     budget = 1 << args.budget_log2
-    iters = max(1, budget // (per_iter * args.steps))
+    # the budget in whole launch iterations, spread over the timed steps so
+    # that exactly `total_iters` x per_iter keys are walked (2^36 at the
+    # default shape: 64 iterations of 2^30 keys); steps get floor or ceil
+    total_iters = max(args.steps, budget // per_iter)
+    step_iters = [total_iters // args.steps + (1 if i < total_iters % args.steps else 0) for i in range(args.steps)]
+    iters = step_iters[0]
+    mean_iters = total_iters / args.steps
     a = lxv.launch_args_for(ed_pub, prefix, iters=iters, max_hits=lxv.MAX_HITS)
 
     for _ in range(args.warmup):
@@ -287,7 +345,8 @@ def main():
 
     kernel_ms, keys, wall, hits = [], 0, 0.0, []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for it in step_iters:
+        a.iters = it
         h, _ = b.launch(a)
         st = b.stats()
         kernel_ms.append(st.kernel_ms)
@@ -324,9 +383,9 @@ def main():
     elif pairs:
         # B/2 prefix products written and read back per batch of B keys, plus
         # the centre (x, y) read and written once per launch
-        alg_bytes_per_key = 2 * fe_bytes / 2 + 2 * 2 * fe_bytes / (iters * b.batch)
+        alg_bytes_per_key = 2 * fe_bytes / 2 + 2 * 2 * fe_bytes / (mean_iters * b.batch)
     else:
-        alg_bytes_per_key = 2 * 2 * fe_bytes + 2 * 4 * fe_bytes / (iters * b.batch)
+        alg_bytes_per_key = 2 * 2 * fe_bytes + 2 * 4 * fe_bytes / (mean_iters * b.batch)
     roofline["hbm"] = {"algorithmic_bytes_per_key": alg_bytes_per_key,
                        "achieved_GBps": value * alg_bytes_per_key / 1e9, "peak_GBps": 8000.0,
                        "frac": value * alg_bytes_per_key / 8e12}
@@ -334,29 +393,30 @@ def main():
     kname, grid_lanes = b.walk_kernel()
     if pmc and kname in pmc.get("kernel", "") and "hbm_bytes_per_key" in pmc:
         # bytes per launch of this run = measured bytes per key x keys per launch
-        roofline["traffic"] = pmc["hbm_bytes_per_key"] * per_iter * iters
+        roofline["traffic"] = pmc["hbm_bytes_per_key"] * keys / len(step_iters)
         roofline["traffic_source"] = "profiles/walk_pmc.json (FETCH_SIZE/WRITE_SIZE, copy-calibrated, same kernel)"
         v = pmc.get("valu_insts_per_key")
-        if v and clock and cus:
-            # VALU issue: wave64 instructions at one per SIMD per 4 clocks,
-            # i.e. CUs x 64 lane-instructions per clock at the measured clock
-            peak_issue = cus * 64 * clock
-            roofline["valu_issue"] = {"valu_insts_per_key": v, "achieved": value * v / 1e12,
-                                      "peak": peak_issue / 1e12, "unit": "T lane-instr/s",
-                                      "frac": value * v / peak_issue,
-                                      "busy_pct_measured": pmc.get("valu_busy_pct")}
+        if v:
+            roofline["valu_issue"] = valu_issue_roofline(value, v, kname, pmc.get("valu_busy_pct"))
+    if mad_peak and "valu_issue" in roofline and roofline["valu_issue"].get("clock_hz_measured"):
+        # the mad probe against one wave64 instruction per SIMD per 4 clocks
+        # at the clock the probes measured, not the nominal one
+        vi = roofline["valu_issue"]
+        roofline["mad_probe_per_simd_4clk"] = mad_peak / (vi["cus"] * 64 * vi["clock_hz_measured"])
 
     out = {
         "metric": METRIC, "value": value, "unit": "keys/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": sum(kernel_ms) / len(kernel_ms), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u32 (exact integer limbs)",
         "data": "synthetic (seeded start scalar, Ed25519 key and prefix)",
-        "config": {"workload": ("plain HIP walk (%s form), single search, 7-hex prefix, B=%d, 2^%d-key budget"
-                                % (args.walk, b.batch, args.budget_log2)) if not args.subwalks else
+        "config": {"workload": ("plain HIP walk (%s form), single search, 7-hex prefix, B=%d, %d keys (%s) in %d "
+                                "launches of %s iterations" % (args.walk, b.batch, keys, keys_text(keys), args.steps,
+                                                               "/".join(str(x) for x in sorted(set(step_iters)))))
+                   if not args.subwalks else
                                ("sub-walk form: %d affine walks per lane inverted together, single search, 7-hex "
                                 "prefix, 2^%d-key budget" % (args.subwalks, args.budget_log2)),
                    "threads": b.threads, "kernel": kname, "grid_lanes": grid_lanes,
-                   "keys_per_inversion": kpi, "iters_per_step": iters,
+                   "keys_per_inversion": kpi, "iters_per_step": step_iters,
                    "prefix": prefix, "layout": "10x25.5" if args.layout == lxv.LAYOUT_10X26 else "8x32",
                    "device": b.describe().strip()},
         "keys": keys, "wall_s": wall, "hits": len(hits), "hits_rechecked": rechecked,

@@ -122,8 +122,7 @@ __global__ __launch_bounds__(256) void mad_rate_kernel(uint32_t* out, uint32_t s
 // Issue-rate probes of single VALU instructions and of the SHA-256 mix, at
 // a chosen occupancy: 8 independent chains per lane (so a wave never waits on
 // its own results), each loop iteration kValuOpsPerIter[op] instructions,
-// exactly as written (register operands only; iters a multiple of 4, the
-// loop's unroll). Thread 0 of every block
+// exactly as written (register operands only). Thread 0 of every block
 // records the shader clock (s_memtime) and the constant-rate clock
 // (s_memrealtime) at its start and end, so the host can tell the clock the
 // SIMDs actually ran at from the kernel itself.
@@ -131,9 +130,10 @@ enum ValuProbeOp {
     kVpAlignbit = 0, kVpBitop3 = 1, kVpAdd3 = 2, kVpAdd = 3, kVpXor = 4, kVpPerm = 5, kVpShift = 6,
     kVpMad64 = 7, kVpShaMix = 8, kVpAddCo = 9, kVpCarryChain = 10, kVpWalkMix = 11, kVpOps = 12
 };
-constexpr int kValuOpsPerIter[kVpOps] = {8, 8, 8, 8, 8, 8, 8, 8, 56, 8, 9, 200};
+constexpr int kValuOpsPerIter[kVpOps] = {32, 32, 32, 32, 32, 32, 32, 32, 56, 32, 36, 200};
 
 #define LXV_R8(I) I(0) I(1) I(2) I(3) I(4) I(5) I(6) I(7)
+#define LXV_R32(I) LXV_R8(I) LXV_R8(I) LXV_R8(I) LXV_R8(I)
 
 template <int kOp>
 __global__ __launch_bounds__(256) void valu_rate_kernel(uint32_t* out, unsigned long long* clocks, uint32_t seed,
@@ -148,53 +148,51 @@ __global__ __launch_bounds__(256) void valu_rate_kernel(uint32_t* out, unsigned 
              x6 = x0 * 13u, x7 = x0 * 15u;
     uint64_t w0 = x0, w1 = x1, w2 = x2, w3 = x3, w4 = x4, w5 = x5, w6 = x6, w7 = x7;
     const uint32_t y = seed + 0x9e3779b9u, z = seed * 0x85ebca6bu;
-#pragma unroll 4
     for (uint32_t it = 0; it < iters; it++) {
         if constexpr (kOp == kVpAlignbit) {
 #define LXV_I(k) "v_alignbit_b32 %" #k ", %" #k ", %8, 7\n"
-            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+            asm volatile(LXV_R32(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
                          "+v"(x7) : "v"(y), "v"(z));
 #undef LXV_I
         } else if constexpr (kOp == kVpBitop3) {
 #define LXV_I(k) "v_bitop3_b32 %" #k ", %" #k ", %8, %9 bitop3:0x96\n"
-            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+            asm volatile(LXV_R32(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
                          "+v"(x7) : "v"(y), "v"(z));
 #undef LXV_I
         } else if constexpr (kOp == kVpAdd3) {
 #define LXV_I(k) "v_add3_u32 %" #k ", %" #k ", %8, %9\n"
-            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+            asm volatile(LXV_R32(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
                          "+v"(x7) : "v"(y), "v"(z));
 #undef LXV_I
         } else if constexpr (kOp == kVpAdd) {
 #define LXV_I(k) "v_add_u32_e32 %" #k ", %8, %" #k "\n"
-            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+            asm volatile(LXV_R32(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
                          "+v"(x7) : "v"(y), "v"(z));
 #undef LXV_I
         } else if constexpr (kOp == kVpXor) {
 #define LXV_I(k) "v_xor_b32_e32 %" #k ", %8, %" #k "\n"
-            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+            asm volatile(LXV_R32(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
                          "+v"(x7) : "v"(y), "v"(z));
 #undef LXV_I
         } else if constexpr (kOp == kVpPerm) {
 #define LXV_I(k) "v_perm_b32 %" #k ", %" #k ", %8, %9\n"
-            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+            asm volatile(LXV_R32(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
                          "+v"(x7) : "v"(y), "v"(z));
 #undef LXV_I
         } else if constexpr (kOp == kVpShift) {
 #define LXV_I(k) "v_lshrrev_b32_e32 %" #k ", %8, %" #k "\n"
-            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+            asm volatile(LXV_R32(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
                          "+v"(x7) : "v"(y), "v"(z));
 #undef LXV_I
         } else if constexpr (kOp == kVpMad64) {
             uint64_t c0_, c1_, c2_, c3_, c4_, c5_, c6_, c7_;
-            asm volatile(
-                "v_mad_u64_u32 %0, %8, %16, %17, %0\n v_mad_u64_u32 %1, %9, %16, %17, %1\n"
-                "v_mad_u64_u32 %2, %10, %16, %17, %2\n v_mad_u64_u32 %3, %11, %16, %17, %3\n"
-                "v_mad_u64_u32 %4, %12, %16, %17, %4\n v_mad_u64_u32 %5, %13, %16, %17, %5\n"
-                "v_mad_u64_u32 %6, %14, %16, %17, %6\n v_mad_u64_u32 %7, %15, %16, %17, %7"
-                : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5), "+v"(w6), "+v"(w7), "=s"(c0_),
-                  "=s"(c1_), "=s"(c2_), "=s"(c3_), "=s"(c4_), "=s"(c5_), "=s"(c6_), "=s"(c7_)
-                : "v"(y), "v"(z));
+#define LXV_I(k) "v_mad_u64_u32 %" #k ", %[c" #k "], %16, %17, %" #k "\n"
+            asm volatile(LXV_R32(LXV_I)
+                         : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5), "+v"(w6), "+v"(w7),
+                           [c0] "=&s"(c0_), [c1] "=&s"(c1_), [c2] "=&s"(c2_), [c3] "=&s"(c3_), [c4] "=&s"(c4_),
+                           [c5] "=&s"(c5_), [c6] "=&s"(c6_), [c7] "=&s"(c7_)
+                         : "v"(y), "v"(z));
+#undef LXV_I
         } else if constexpr (kOp == kVpShaMix) {
             // per chain: 3 rotates, 2 three-input boolean functions, 2
             // three-input adds (the proportions of a SHA-256 round's
@@ -215,7 +213,7 @@ __global__ __launch_bounds__(256) void valu_rate_kernel(uint32_t* out, unsigned 
 #undef LXV_M
         } else if constexpr (kOp == kVpAddCo) {
 #define LXV_I(k) "v_add_co_u32_e32 %" #k ", vcc, %8, %" #k "\n"
-            asm volatile(LXV_R8(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+            asm volatile(LXV_R32(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
                          "+v"(x7) : "v"(y), "v"(z) : "vcc");
 #undef LXV_I
         } else if constexpr (kOp == kVpWalkMix) {
@@ -224,38 +222,40 @@ __global__ __launch_bounds__(256) void valu_rate_kernel(uint32_t* out, unsigned 
             // v_bitop3, 12% v_add3, 9% v_addc_co_u32_e64, 8% v_mad_u64_u32, 8%
             // v_add_u32, 6% v_mov, 3% v_lshrrev), 25 per chain per iteration
             uint64_t cout_;
-#define LXV_A(k) "v_alignbit_b32 %" #k ", %" #k ", %" #k ", 6\n"
-#define LXV_B(k) "v_bitop3_b32 %" #k ", %" #k ", %17, %18 bitop3:0x96\n"
-#define LXV_C(k) "v_add3_u32 %" #k ", %" #k ", %17, %18\n"
-#define LXV_D(k) "v_addc_co_u32_e64 %" #k ", %16, %" #k ", %17, %19\n"
-#define LXV_E(k) "v_mad_u64_u32 %[w" #k "], %16, %17, %18, %[w" #k "]\n"
-#define LXV_F(k) "v_add_u32_e32 %" #k ", %17, %" #k "\n"
-#define LXV_G(k) "v_mov_b32_e32 %" #k ", %18\n"
-#define LXV_H(k) "v_lshrrev_b32_e32 %" #k ", %17, %" #k "\n"
-            asm volatile(LXV_R8(LXV_A) LXV_R8(LXV_A) LXV_R8(LXV_A) LXV_R8(LXV_B) LXV_R8(LXV_B) LXV_R8(LXV_C)
-                         LXV_R8(LXV_A) LXV_R8(LXV_A) LXV_R8(LXV_A) LXV_R8(LXV_B) LXV_R8(LXV_C) LXV_R8(LXV_D)
-                         LXV_R8(LXV_E) LXV_R8(LXV_F) LXV_R8(LXV_G) LXV_R8(LXV_A) LXV_R8(LXV_A) LXV_R8(LXV_B)
-                         LXV_R8(LXV_B) LXV_R8(LXV_C) LXV_R8(LXV_D) LXV_R8(LXV_E) LXV_R8(LXV_F) LXV_R8(LXV_G)
-                         LXV_R8(LXV_H)
+#define LXV_PA(k) "v_alignbit_b32 %" #k ", %" #k ", %" #k ", 6\n"
+#define LXV_PB(k) "v_bitop3_b32 %" #k ", %" #k ", %17, %18 bitop3:0x96\n"
+#define LXV_PC(k) "v_add3_u32 %" #k ", %" #k ", %17, %18\n"
+#define LXV_PD(k) "v_addc_co_u32_e64 %" #k ", %16, %" #k ", %17, %19\n"
+#define LXV_PE(k) "v_mad_u64_u32 %[w" #k "], %16, %17, %18, %[w" #k "]\n"
+#define LXV_PF(k) "v_add_u32_e32 %" #k ", %17, %" #k "\n"
+#define LXV_PG(k) "v_mov_b32_e32 %" #k ", %18\n"
+#define LXV_PH(k) "v_lshrrev_b32_e32 %" #k ", %17, %" #k "\n"
+            asm volatile(LXV_R8(LXV_PA) LXV_R8(LXV_PA) LXV_R8(LXV_PA) LXV_R8(LXV_PB) LXV_R8(LXV_PB) LXV_R8(LXV_PC)
+                         LXV_R8(LXV_PA) LXV_R8(LXV_PA) LXV_R8(LXV_PA) LXV_R8(LXV_PB) LXV_R8(LXV_PC) LXV_R8(LXV_PD)
+                         LXV_R8(LXV_PE) LXV_R8(LXV_PF) LXV_R8(LXV_PG) LXV_R8(LXV_PA) LXV_R8(LXV_PA) LXV_R8(LXV_PB)
+                         LXV_R8(LXV_PB) LXV_R8(LXV_PC) LXV_R8(LXV_PD) LXV_R8(LXV_PE) LXV_R8(LXV_PF) LXV_R8(LXV_PG)
+                         LXV_R8(LXV_PH)
                          : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6), "+v"(x7),
                            [w0] "+v"(w0), [w1] "+v"(w1), [w2] "+v"(w2), [w3] "+v"(w3), [w4] "+v"(w4), [w5] "+v"(w5),
                            [w6] "+v"(w6), [w7] "+v"(w7), "=&s"(cout_)
                          : "v"(y), "v"(z), "s"((uint64_t)0));
-#undef LXV_A
-#undef LXV_B
-#undef LXV_C
-#undef LXV_D
-#undef LXV_E
-#undef LXV_F
-#undef LXV_G
-#undef LXV_H
+#undef LXV_PA
+#undef LXV_PB
+#undef LXV_PC
+#undef LXV_PD
+#undef LXV_PE
+#undef LXV_PF
+#undef LXV_PG
+#undef LXV_PH
         } else {
             // one dependent carry chain through VCC (the shape of a field
             // addition): each instruction reads the carry the previous wrote
 #define LXV_I(k) "v_addc_co_u32_e32 %" #k ", vcc, %8, %" #k ", vcc\n"
-            asm volatile("v_add_co_u32_e32 %0, vcc, %8, %0\n" LXV_R8(LXV_I)
+#define LXV_C9 "v_add_co_u32_e32 %0, vcc, %8, %0\n" LXV_R8(LXV_I)
+            asm volatile(LXV_C9 LXV_C9 LXV_C9 LXV_C9
                          : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6), "+v"(x7)
                          : "v"(y), "v"(z) : "vcc");
+#undef LXV_C9
 #undef LXV_I
         }
     }
@@ -268,6 +268,7 @@ __global__ __launch_bounds__(256) void valu_rate_kernel(uint32_t* out, unsigned 
         c[3] = wall_clock64();
     }
 }
+#undef LXV_R32
 #undef LXV_R8
 
 // A plain copy of known size, for scaling HBM byte counters.

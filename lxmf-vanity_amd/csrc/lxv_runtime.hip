@@ -642,7 +642,7 @@ extern "C" int lxv_debug_valu_rate(int device, int op, uint32_t waves_per_cu, do
         return LXV_E_HIP;
     // one resident round at the asked occupancy: blocks of 4 waves
     const uint32_t blocks = (uint32_t)prop.multiProcessorCount * (waves_per_cu / 4);
-    const uint32_t lanes = blocks * 256u, iters = 16384, runs = 5;
+    const uint32_t lanes = blocks * 256u, iters = 4096, runs = 5;
     uint32_t* out = nullptr;
     unsigned long long* clocks = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
