@@ -158,6 +158,28 @@ int lxv_launch(lxv_ctx* ctx, const lxv_launch_args* args);
 int lxv_poll(lxv_ctx* ctx, int wait, lxv_hit* hits, uint32_t cap, uint32_t* n_hits,
              uint32_t* dump, size_t dump_words);
 
+/* No counterpart in the reference. The launch of lxv_launch for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc. It allocates nothing, copies
+ * nothing and does not wait: it enqueues two clearing kernels and the walk
+ * kernel on `stream` (a hipStream_t; NULL = the context's own stream), so it
+ * may be captured into a graph. The walk is the context's own (same kernel,
+ * grid and lane state, updated in place as by lxv_launch); the caller passes
+ * the outputs in device memory: d_hits (max_hits records, may be NULL when
+ * max_hits is 0) and d_hit_count (one word, set to the matches found, which
+ * may exceed max_hits). Its scratch (the launch counters and the batch
+ * arrays of the grid) is a 32 MiB static device array per GPU, held from the
+ * first lxv_open on that GPU until the process ends; contexts whose grid
+ * needs more (resident lanes x batch / 2 x 32 bytes for 8 x 32 limbs, e.g.
+ * more than 32768 resident lanes at batch 64) are refused with LXV_E_ARG, as
+ * are the projective and sub-walk forms and dump_threads > 0. All calls on
+ * one GPU share that scratch: calls that overlap on one GPU (two streams,
+ * or a launch of this kind still running) race. No lxv_poll follows: the
+ * caller waits on its stream and reads its outputs itself; the context's
+ * lxv_last_stats are not updated. */
+int lxv_launch_noalloc(lxv_ctx* ctx, const lxv_launch_args* args, lxv_hit* d_hits, uint32_t* d_hit_count,
+                       void* stream);
+
 /* Asks a running launch to end early: every lane finishes the batch it is
  * in, stores its state and exits, so the walk can resume with the next
  * launch. No counterpart in the reference's trait; it is the "stop" of the

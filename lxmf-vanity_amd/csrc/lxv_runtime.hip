@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -32,7 +33,47 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// Scratch of lxv_launch_noalloc: a file-scope device array, so that entry
+// point allocates nothing. Words [0, 64) hold the launch counters (queue,
+// stopped flag, batches walked), the rest the pair walk's batch arrays of
+// the context's grid. Each GPU holds its own copy from the first use of this
+// code object until the process ends. Host code never uses the array's name
+// as an address: g_noalloc.ptr[device] is its device address, from
+// hipGetSymbolAddress on that device.
+constexpr size_t kNoAllocCounterWords = 64;
+constexpr size_t kNoAllocBatchBytes = (size_t)32 << 20;
+constexpr size_t kNoAllocWords = kNoAllocCounterWords + kNoAllocBatchBytes / sizeof(uint32_t);
+constexpr int kNoAllocMaxDevices = 64;
+
+struct NoAllocScratch {
+    std::mutex mu;
+    uint32_t* ptr[kNoAllocMaxDevices] = {};
+    hipError_t err[kNoAllocMaxDevices] = {};
+};
+NoAllocScratch g_noalloc;
+
 }  // namespace
+
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+// Looks up (once per device, with that device current) the device address of
+// `scratch` on `device`.
+static hipError_t noalloc_scratch(int device, uint32_t** out) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_noalloc.mu);
+    if (!g_noalloc.ptr[device]) {
+        if (g_noalloc.err[device] != hipSuccess) return g_noalloc.err[device];
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            g_noalloc.err[device] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+            return g_noalloc.err[device];
+        }
+        g_noalloc.ptr[device] = static_cast<uint32_t*>(p);
+    }
+    *out = g_noalloc.ptr[device];
+    return hipSuccess;
+}
 
 struct lxv_ctx {
     int device = 0;
@@ -128,6 +169,12 @@ extern "C" int lxv_open(const lxv_open_opts* opts, lxv_ctx** out) {
     if ((e = hipSetDevice(c->device)) != hipSuccess || (e = hipGetDeviceProperties(&prop, c->device)) != hipSuccess) {
         delete c;
         return fail(nullptr, LXV_E_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+    }
+    {
+        // the no-allocation launch's scratch on this device (device current);
+        // a failure here is reported by lxv_launch_noalloc, not by open
+        uint32_t* unused = nullptr;
+        (void)noalloc_scratch(c->device, &unused);
     }
     c->layout = o.layout;  // LXV_LAYOUT_8X32 (0) is the default: the faster layout measured on gfx950
     if (c->layout != LXV_LAYOUT_10X26 && c->layout != LXV_LAYOUT_8X32) {
@@ -434,6 +481,63 @@ extern "C" int lxv_poll(lxv_ctx* c, int wait, lxv_hit* hits, uint32_t cap, uint3
         if (dump_words < dw) return fail(c, LXV_E_ARG, "lxv_poll: dump buffer too small");
         HIPCHK(c, hipMemcpy(dump, c->d_dump, dw * sizeof(uint32_t), hipMemcpyDeviceToHost));
     }
+    return LXV_OK;
+}
+
+extern "C" int lxv_launch_noalloc(lxv_ctx* c, const lxv_launch_args* a, lxv_hit* d_hits, uint32_t* d_hit_count,
+                                  void* stream) {
+    int rc = check_idle(c, "lxv_launch_noalloc");
+    if (rc) return rc;
+    if (!a || !d_hit_count || (a->max_hits && !d_hits))
+        return fail(c, LXV_E_ARG, "lxv_launch_noalloc: args, d_hit_count and (for max_hits > 0) d_hits are needed");
+    if (!c->initialized) return fail(c, LXV_E_STATE, "lxv_launch_noalloc: lanes not initialised (lxv_init first)");
+    if (c->subwalks || c->walk != LXV_WALK_PAIRS)
+        return fail(c, LXV_E_ARG, "lxv_launch_noalloc: only the pair walk (LXV_WALK_PAIRS, no sub-walks) is supported");
+    if (a->dump_threads) return fail(c, LXV_E_ARG, "lxv_launch_noalloc: dump_threads must be 0 (no dump buffer)");
+    const uint32_t grid = c->resident ? c->resident : c->phys;
+    const size_t need = (size_t)grid * (c->batch / 2) * c->limbs * sizeof(uint32_t);
+    if (need > kNoAllocBatchBytes) {
+        char msg[256];
+        snprintf(msg, sizeof msg,
+                 "lxv_launch_noalloc: the batch arrays of this context's grid (%u lanes x %u keys per inversion) "
+                 "need %zu bytes, more than the %zu of the static scratch; open the context with fewer "
+                 "resident_lanes or a smaller batch",
+                 grid, c->batch, need, kNoAllocBatchBytes);
+        return fail(c, LXV_E_ARG, msg);
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    uint32_t* scr = nullptr;
+    HIPCHK(c, noalloc_scratch(c->device, &scr));
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    WalkParams p;
+    std::memset(&p, 0, sizeof p);
+    sha_search_init(p.sha, a->ed, a->name_hash, a->want, a->mask);
+    fill_q(c->layout, a->q, p.qp, p.qm, p.q);
+    p.state = c->d_state;
+    p.ns = c->d_ns;
+    p.xor_acc = c->d_xor;
+    p.batch_mem = reinterpret_cast<uint2*>(scr + kNoAllocCounterWords);
+    p.hits = d_hits;
+    p.hit_count = d_hit_count;
+    p.stopped = scr + 1;
+    p.batches = reinterpret_cast<unsigned long long*>(scr + 2);
+    p.queue = scr + 4;
+    p.dump = nullptr;
+    p.stop = c->d_stop;
+    p.max_hits = a->max_hits;
+    p.dump_threads = 0;
+    p.lanes = c->phys;
+    p.iters = a->iters;
+    p.resident = c->resident;
+    *(volatile uint32_t*)c->h_stop = 0;
+    // the scratch keeps the last call's contents: clear its counters, and
+    // the caller's hit count, with a kernel (capturable, unlike a memset)
+    HIPCHK(c, launch_clear(scr, (uint32_t)kNoAllocCounterWords, s));
+    HIPCHK(c, launch_clear(d_hit_count, 1, s));
+    if (c->resident)
+        HIPCHK(c, launch_walk_pair_queue((int)c->layout, (int)c->batch, (int)c->waves, p, s));
+    else
+        HIPCHK(c, launch_walk_pair((int)c->layout, (int)c->batch, (int)c->waves, p, s));
     return LXV_OK;
 }
 
