@@ -210,9 +210,12 @@ def pmc_summary(out_dir, dest=os.path.join(ROOT, "profiles", "walk_pmc.json")):
     return out
 
 
-def plain_rate(batch=256, waves_per_cu=16, budget_log2=33):
+def plain_rate(batch=0, waves_per_cu=0, budget_log2=33):
     """keys/s of the plain kernel on one search (workload seed 2, 8 hex
-    digits), kernel time only: the yardstick for the many-search workloads."""
+    digits) at the tuned single-search shape, kernel time only: the yardstick
+    for the many-search workloads."""
+    single = lxv.tuned_shape("single")
+    batch, waves_per_cu = batch or single["batch"], waves_per_cu or single["waves_per_cu"]
     start, ed_seed, prefix = lxv.seeded_search(2, 8)
     s = lxv.GpuSearch(start=start, ed_seed=ed_seed, batch=batch, waves_per_cu=waves_per_cu)
     try:
@@ -231,11 +234,37 @@ def plain_rate(batch=256, waves_per_cu=16, budget_log2=33):
         s.close()
 
 
+def serving_pass(args, plain_keys_per_s):
+    """The metric's second half, in the same run: the serving loop over the
+    Poisson request stream (20/s, 5-9 hex digits) for --serving-s seconds,
+    p50/p99 time-to-hit from arrival over the finished requests and over all
+    of them (unfinished counted at their age), and the same arrivals replayed
+    one by one on the plain kernel for --one-by-one-s seconds."""
+    import workloads
+    log = lambda msg: print(msg, file=sys.stderr, flush=True)  # noqa: E731
+    r = workloads.serving(duration_s=args.serving_s, policy=args.policy, oldest_share_pct=args.oldest_share_pct,
+                          progress=log)
+    keep = ("requests", "finished", "unfinished", "p50_s", "p99_s", "p50_s_with_unfinished", "p99_s_with_unfinished",
+            "policy", "oldest_share_pct", "shape", "keys_per_s", "busy_s", "steps", "rechecked", "per_length")
+    out = {k: r[k] for k in keep}
+    out["workload"] = ("serving: Poisson 20/s for %.0f s, 5-9 hex digits (P=.30/.30/.20/.15/.05), a fresh seeded "
+                       "identity per request, admission %s" % (args.serving_s, args.policy))
+    out["keys_per_s_vs_plain"] = r["keys_per_s"] / plain_keys_per_s
+    out["queue_depth_end"] = r["queue_depth"][-1] if r["queue_depth"] else None
+    if args.one_by_one_s > 0:
+        o = workloads.one_by_one(duration_s=args.one_by_one_s, progress=log)
+        out["one_by_one"] = {k: o[k] for k in ("requests", "finished", "unfinished", "p50_s", "p99_s",
+                                               "p50_s_with_unfinished", "p99_s_with_unfinished", "keys_per_s",
+                                               "policy")}
+    return out
+
+
 def many_search_main(args):
     import workloads
     log = lambda msg: print(msg, file=sys.stderr, flush=True)  # noqa: E731
     base = plain_rate()
-    log("plain kernel, one search (B=256, 16 waves/CU): %.3e keys/s" % base)
+    single = lxv.tuned_shape("single")
+    log("plain kernel, one search (B=%d, %d waves/CU): %.3e keys/s" % (single["batch"], single["waves_per_cu"], base))
     common = {"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
               "dtype": "u32 (exact integer limbs)",
               "data": "synthetic (seeded start scalars, Ed25519 keys and prefixes)", "plain_keys_per_s": base}
@@ -248,7 +277,7 @@ def many_search_main(args):
                "steps": r["steps"], "warmup": 0, "ms_per_step": 1e3 * r["wall_s"] / max(1, r["steps"]),
                "config": {"workload": "batched: 256 seeded searches, 4-10 hex digits "
                                       "(P=.10/.15/.20/.20/.15/.12/.08), %.0f s wall" % args.wall_s,
-                          "keys_per_inversion": 256, "layout": "8x32"},
+                          "shape": r["shape"]},
                "vs_plain": r["keys_per_s"] / base, "kernel_vs_plain": (r["kernel_keys_per_s"] or 0) / base}
         out.update(common)
         out["result"] = r
@@ -259,22 +288,23 @@ def many_search_main(args):
                "p50_s": r["p50_s"], "p99_s": r["p99_s"], "unfinished": r["unfinished"],
                "config": {"workload": "serving arrivals: Poisson 20/s for %.0f s, 5-9 hex digits "
                                       "(P=.30/.30/.20/.15/.05), one search at a time in arrival order"
-                                      % args.duration_s, "keys_per_inversion": 256, "layout": "8x32"}}
+                                      % args.duration_s, "shape": lxv.tuned_shape("single")}}
         out.update(common)
         out["result"] = r
     else:
         kw = {"lane_keys_log2": args.lane_keys_log2} if args.lane_keys_log2 else {}
         if args.budget_ms:
             kw["budget_ms"] = args.budget_ms
-        r = workloads.serving(duration_s=args.duration_s, policy=args.policy, progress=log, **kw)
+        r = workloads.serving(duration_s=args.duration_s, policy=args.policy, progress=log,
+                              oldest_share_pct=args.oldest_share_pct, **kw)
         out = {"metric": "p50/p99 time-to-hit, mixed prefixes (serving)", "value": r["keys_per_s"],
                "unit": "keys/s", "steps": r["steps"], "warmup": 0,
                "ms_per_step": 1e3 * r["wall_s"] / max(1, r["steps"]),
                "p50_s": r["p50_s"], "p99_s": r["p99_s"], "unfinished": r["unfinished"],
                "config": {"workload": "serving: Poisson 20/s for %.0f s, 5-9 hex digits (P=.30/.30/.20/.15/.05), "
-                                      "admission %s, %d ms steps, 2^%d keys per lane" % (args.duration_s, args.policy, r["budget_ms"],
-                                                                          r["lane_keys_log2"]),
-                          "keys_per_inversion": 256, "layout": "8x32"}}
+                                      "admission %s, %d ms steps, 2^%d keys per lane"
+                                      % (args.duration_s, args.policy, r["budget_ms"], r["lane_keys_log2"]),
+                          "shape": r["shape"]}}
         out.update(common)
         out["result"] = r
         if args.one_by_one:
@@ -288,7 +318,13 @@ def main():
                     help="one-by-one: the serving arrivals through the plain kernel alone, one search at a time")
     ap.add_argument("--wall-s", type=float, default=60.0)
     ap.add_argument("--duration-s", type=float, default=60.0)
-    ap.add_argument("--policy", choices=("short-first", "fifo"), default="short-first")
+    ap.add_argument("--policy", choices=("short-first", "short-first-oldest", "fifo"), default="short-first-oldest")
+    ap.add_argument("--oldest-share-pct", type=int, default=20,
+                    help="short-first-oldest: share of every step for the longest-waiting request")
+    ap.add_argument("--serving-s", type=float, default=60.0,
+                    help="plain workload: also run the serving loop for this long and report time-to-hit (0: skip)")
+    ap.add_argument("--one-by-one-s", type=float, default=60.0,
+                    help="plain workload: also replay the serving arrivals one by one on the plain kernel (0: skip)")
     ap.add_argument("--one-by-one", action="store_true")
     ap.add_argument("--lane-keys-log2", type=int, default=0,
                     help="serving/batched: expected keys per virtual lane, log2 (0: the workload's default)")
@@ -430,10 +466,12 @@ def main():
         ms = ctypes.c_double(0)
         L.lxv_debug_copy(0, args.copy_probe_mib << 20, ctypes.byref(ms))
         out["copy_probe"] = {"bytes": args.copy_probe_mib << 20, "ms": ms.value}
+    search.close()
+    if args.serving_s > 0:
+        out["time_to_hit"] = serving_pass(args, value)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
-    search.close()
 
 
 if __name__ == "__main__":

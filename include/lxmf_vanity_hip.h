@@ -67,8 +67,9 @@ typedef struct lxv_open_opts {
                               pair walk, whose resident waves take 64-lane groups from a
                               queue; R = 8 for the projective walk (rounds of waves for
                               the dispatcher); R = 1 for the sub-walk form */
-    uint32_t batch;        /* keys per batched inversion (B); 0 = 64 */
-    uint32_t waves_per_cu; /* resident waves per CU the grid is sized for; 0 = 16 */
+    uint32_t batch;        /* keys per batched inversion (B); 0 = the tuned pick (LXV_TUNED_SINGLE) */
+    uint32_t waves_per_cu; /* resident waves per CU the grid is sized for: 4, 8 or 16 (1, 2 or 4
+                              per SIMD); 0 = the tuned pick (LXV_TUNED_SINGLE) */
     uint32_t layout;       /* LXV_LAYOUT_*; 0 (8x32) is the default */
     uint32_t subwalks;     /* 0: one walk per lane, B keys per inversion (the reference's
                               form). K = 64, 128 or 256 (8x32 only): K walks per lane in
@@ -152,9 +153,11 @@ int lxv_launch(lxv_ctx* ctx, const lxv_launch_args* args);
  * Backend::launch (src/gpu/cuda.rs:124-131). With wait = 0 it returns
  * LXV_E_BUSY while the kernel runs; with wait != 0 it blocks. Fills up to
  * `cap` hits (order unspecified, at most max_hits exist) and sets *n_hits;
- * `dump` (dump_words >= dump_threads x B x 8, or NULL when dump_threads was
- * 0) receives the first batch's u words of the dumped lanes, lane-major as
- * the reference's dump (src/gpu/common.h:340-343). */
+ * `dump` (dump_words >= dump_threads x B x 8, or NULL to skip it) receives
+ * the first batch's u words of the dumped lanes, lane-major as the
+ * reference's dump (src/gpu/common.h:340-343). A dump buffer that is too
+ * small fails with LXV_E_ARG before anything is collected: the launch stays
+ * in flight and the call can be repeated. */
 int lxv_poll(lxv_ctx* ctx, int wait, lxv_hit* hits, uint32_t cap, uint32_t* n_hits,
              uint32_t* dump, size_t dump_words);
 
@@ -204,6 +207,29 @@ void lxv_close(lxv_ctx* ctx);
 /* The message of the last failure on ctx (or of the last failed lxv_open
  * when ctx is NULL); never NULL. The String side of Result<_, String>. */
 const char* lxv_last_error(lxv_ctx* ctx);
+
+/* The autotuner's launch shapes, compiled into the library from its table
+ * (tuned.json): what lxv_open uses for batch = 0 and waves_per_cu = 0
+ * (LXV_TUNED_SINGLE), what lxv_multi_open uses for batch = 0 and
+ * waves_per_cu = 0 (LXV_TUNED_BATCHED), and the shape a serving loop should
+ * open lxv_multi with (LXV_TUNED_SERVING). No counterpart in the reference,
+ * whose launch shape is fixed by hand (src/gpu/cuda.rs:12-16). */
+#define LXV_TUNED_SINGLE 0   /* one search, the plain kernel */
+#define LXV_TUNED_BATCHED 1  /* many searches submitted at once */
+#define LXV_TUNED_SERVING 2  /* a stream of requests arriving over time */
+
+typedef struct lxv_tuned_shape {
+    uint32_t batch;          /* keys per inversion */
+    uint32_t waves_per_cu;   /* resident waves per CU */
+    uint32_t layout;         /* LXV_LAYOUT_* */
+    uint32_t walk;           /* LXV_WALK_* */
+    uint32_t step_ms;        /* batched and serving: the step budget (lxv_multi_step); 0 for a single search */
+    uint32_t lane_keys_log2; /* batched and serving: lxv_multi_opts.lane_keys_log2; 0 for a single search */
+    double keys_per_s;       /* what the tuner measured at this shape (0: not measured) */
+} lxv_tuned_shape;
+
+/* The compiled-in shape for `workload` (LXV_TUNED_*): LXV_OK or LXV_E_ARG. */
+int lxv_tuned(int workload, lxv_tuned_shape* out);
 
 /* ------------------------------------------------------------------------
  * Host helpers of the search driver (the parts of GpuSearch and main.rs the
@@ -264,11 +290,14 @@ typedef struct lxv_multi lxv_multi;
 
 typedef struct lxv_multi_opts {
     uint32_t size;          /* sizeof(lxv_multi_opts); a struct ending at lane_keys_log2
-                               (an older header) is accepted, `walk` then defaults */
+                               (an older header) is accepted, the later fields then
+                               default */
     int32_t device;         /* HIP device ordinal; negative means 0 */
     uint32_t max_searches;  /* slots resident at once; 0 = 1024 */
-    uint32_t batch;         /* keys per inversion: 64 or 256; 0 = 256 */
-    uint32_t layout;        /* LXV_LAYOUT_*: 256 supports both, 64 only 8x32 */
+    uint32_t batch;         /* keys per inversion: pair walk 64, 128 or 256 (8x32),
+                               256 (10x25.5); projective 64 or 256 (8x32), 256
+                               (10x25.5); 0 = the tuned pick (LXV_TUNED_BATCHED) */
+    uint32_t layout;        /* LXV_LAYOUT_* */
     uint32_t no_graph;      /* 1: launch eagerly instead of replaying a hipGraph */
     uint64_t pool_bytes;    /* device bytes for resident walk state; 0 = 2 GiB */
     uint32_t max_vlanes;    /* cap on a search's virtual lanes; 0 = 262144 */
@@ -285,6 +314,14 @@ typedef struct lxv_multi_opts {
                                back from the following call (at the same time on the
                                clock: the call waits for that step). 0 (default):
                                each call runs one step to completion */
+    uint32_t waves_per_cu;  /* resident waves per CU the kernel is built for: 8 or 16
+                               (the pair walk; the projective walk only 16); 0 = the
+                               tuned pick (LXV_TUNED_BATCHED) */
+    uint32_t oldest_share_pct; /* 0..90: share of each step's work given to the
+                               longest-waiting resident search before the rest is
+                               shared by priority, and that search is admitted first;
+                               bounds how long a low-priority search can starve.
+                               0 (default): strict priority */
 } lxv_multi_opts;
 
 typedef struct lxv_search_spec {

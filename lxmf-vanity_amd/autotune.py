@@ -1,5 +1,10 @@
-"""Autotuner of the walk kernel: keys per inversion x waves per CU x limb layout
-(x walk form: the pair walk and the reference's projective walk).
+"""Autotuner of the walk kernels for three workloads: a single search on the
+plain kernel (keys per inversion x waves per CU x limb layout x walk form),
+the batched mix of 256 searches, and the serving stream (the batched kernel's
+keys per inversion x waves per CU x layout, and for serving the step budget).
+The picks go to tuned.json's "picks"; the library is built with them
+(gen_tuned.py), so they are the default shapes of lxv_open, lxv_multi_open,
+the command-line tool and the serving loop.
 
 Each point opens the backend with that shape at the library's own launch
 shape (--shape grid, the default): the pair walk on a persistent grid of
@@ -20,6 +25,14 @@ merged, VALU busy %. The fastest point is the pick; the table goes to JSON.
     python autotune.py --merge-pmc OUT/pmc_counter_collection.csv --out tuned.json
     # separate sweeps (say one per walk form) joined, picked anew and merged
     python autotune.py --join a.json b.json [--merge-pmc CSV] --out tuned.json
+    # the batched mix and the serving stream, each point run for --wall-s
+    # seconds on its workload; picks merged into --out
+    python autotune.py --mix batched [--wall-s 20] --out tuned.json
+    python autotune.py --mix serving [--wall-s 20] --out tuned.json
+
+Mix picks: batched, the highest aggregate keys/s; serving, the lowest p50
+time-to-hit over all requests (unfinished counted at their age) among the
+points within 10% of the best keys/s, then the highest keys/s.
 
 VALU busy % = SQ_ACTIVE_INST_VALU x 4 / (SIMDs x GRBM_GUI_ACTIVE / XCDs): the
 share of SIMD cycles issuing VALU work (a wave64 instruction occupies a
@@ -140,7 +153,15 @@ def kernel_key(name, grid):
     return None
 
 
-def merge_pmc(table, csv_path):
+def pmc_pass_keys(point, budget_log2):
+    """Keys the profiled pass (run_point at 2^budget_log2) walked at a point:
+    one warm-up launch of one iteration, then two of `iters`."""
+    per_iter = point["lanes"] * point["batch"]
+    iters = max(1, (1 << budget_log2) // (per_iter * 2))
+    return per_iter * (1 + 2 * iters)
+
+
+def merge_pmc(table, csv_path, budget_log2=28):
     acc = {}
     for r in csv.DictReader(open(csv_path)):
         k = kernel_key(r["Kernel_Name"], r["Grid_Size"])
@@ -158,7 +179,9 @@ def merge_pmc(table, csv_path):
         cycles = d["GRBM_GUI_ACTIVE"] / XCDS
         p["valu_busy_pct"] = 100.0 * d["SQ_ACTIVE_INST_VALU"] * 4 / (cus * SIMDS_PER_CU * cycles)
         if "SQ_INSTS_VALU" in d:
-            p["valu_insts_per_wave_key"] = d["SQ_INSTS_VALU"]
+            # wave instructions x 64 lanes over the keys of the profiled pass
+            p["valu_insts_per_key"] = d["SQ_INSTS_VALU"] * 64 / pmc_pass_keys(p, budget_log2)
+            p.pop("valu_insts_per_wave_key", None)
     return table
 
 
@@ -176,6 +199,55 @@ def pick(table):
     return table
 
 
+MIX_SHAPES = ([dict(walk=lxv.WALK_PAIRS, layout=lxv.LAYOUT_8X32, batch=b, waves_per_cu=w)
+               for b in (64, 128, 256) for w in (8, 16)] +
+              [dict(walk=lxv.WALK_PAIRS, layout=lxv.LAYOUT_10X26, batch=256, waves_per_cu=w) for w in (8, 16)])
+SERVING_STEPS_MS = (15, 25, 50)
+
+
+def mix_sweep(kind, wall_s, progress=print):
+    """Runs the batched mix or the serving stream at every batched shape (and,
+    for serving, every step budget) for wall_s seconds each."""
+    import workloads
+    points = []
+    for sh in MIX_SHAPES:
+        steps = SERVING_STEPS_MS if kind == "serving" else (0,)
+        for step_ms in steps:
+            shape = {"batch": sh["batch"], "waves_per_cu": sh["waves_per_cu"], "layout": sh["layout"],
+                     "walk": sh["walk"]}
+            t0 = time.time()
+            if kind == "batched":
+                r = workloads.batched(wall_s=wall_s, shape=shape, progress=None)
+                pt = {"keys_per_s": r["keys_per_s"], "found": r["found"]}
+            else:
+                shape["step_ms"] = step_ms
+                r = workloads.serving(duration_s=wall_s, shape=shape, progress=None)
+                pt = {k: r[k] for k in ("keys_per_s", "p50_s", "p99_s", "p50_s_with_unfinished",
+                                        "p99_s_with_unfinished", "finished", "unfinished", "policy")}
+            pt.update({"batch": sh["batch"], "waves_per_cu": sh["waves_per_cu"], "layout_id": sh["layout"],
+                       "walk_id": sh["walk"], "layout": "8x32" if sh["layout"] == lxv.LAYOUT_8X32 else "10x25.5",
+                       "step_ms": r["shape"]["step_ms"], "lane_keys_log2": r["shape"]["lane_keys_log2"]})
+            points.append(pt)
+            if progress:
+                progress("%s %-7s B=%-3d waves/CU=%-2d step %3d ms: %.3e keys/s%s  (%.0f s)" % (
+                    kind, pt["layout"], pt["batch"], pt["waves_per_cu"], pt["step_ms"], pt["keys_per_s"],
+                    "" if kind == "batched" else ", p50 %.3f s (all %.3f s)" % (pt["p50_s"] or -1,
+                                                                              pt["p50_s_with_unfinished"] or -1),
+                    time.time() - t0))
+    return points
+
+
+def mix_pick(kind, points):
+    if kind == "batched":
+        best = max(points, key=lambda p: p["keys_per_s"])
+    else:
+        top = max(p["keys_per_s"] for p in points)
+        near = [p for p in points if p["keys_per_s"] >= 0.9 * top and p["p50_s_with_unfinished"] is not None]
+        best = min(near, key=lambda p: (p["p50_s_with_unfinished"], -p["keys_per_s"]))
+    keys = ("batch", "waves_per_cu", "layout_id", "walk_id", "layout", "step_ms", "lane_keys_log2", "keys_per_s")
+    return {k: best[k] for k in keys}
+
+
 def load(path=os.path.join(HERE, "tuned.json")):
     """The tuned table, or None."""
     try:
@@ -190,24 +262,43 @@ def main():
     ap.add_argument("--budget-log2", type=int, default=34)
     ap.add_argument("--out", default=os.path.join(HERE, "tuned.json"))
     ap.add_argument("--merge-pmc", default=None, help="PMC csv of a profiled pass to merge into --out")
+    ap.add_argument("--pmc-budget-log2", type=int, default=28, help="the --budget-log2 of the profiled pass")
+    ap.add_argument("--mix", choices=("batched", "serving"), default=None,
+                    help="sweep the batched kernel's shapes on this workload and merge its pick into --out")
+    ap.add_argument("--wall-s", type=float, default=20.0, help="--mix: seconds per point")
     ap.add_argument("--join", nargs="+", default=None,
                     help="tables of separate sweeps (e.g. one per walk form) to join into --out, picking anew")
     ap.add_argument("--walk", choices=("all", "pairs", "projective"), default="all")
     ap.add_argument("--shape", choices=("grid", "round"), default="grid",
                     help="grid: the library's launch shape at each point's occupancy; round: one resident round")
     args = ap.parse_args()
+    if args.mix:
+        table = load(args.out) or {"points": []}
+        pts = mix_sweep(args.mix, args.wall_s)
+        table.setdefault("mix_points", {})[args.mix] = {
+            "workload": ("batched: 256 seeded searches (4-10 hex digits)" if args.mix == "batched" else
+                         "serving: Poisson 20/s, 5-9 hex digits, short-first-oldest admission") +
+                        ", %.0f s per point" % args.wall_s, "points": pts}
+        table.setdefault("picks", {})[args.mix] = mix_pick(args.mix, pts)
+        if "pick" in table:
+            table["picks"]["single"] = dict(table["pick"])
+        if args.out != "/dev/null":
+            with open(args.out, "w") as f:
+                json.dump(table, f, indent=1)
+        print(json.dumps(table["picks"][args.mix]))
+        return
     if args.join:
         tables = [load(p) for p in args.join]
         if any(t is None for t in tables):
             raise SystemExit("cannot read every table of %s" % args.join)
         table = pick({"workload": tables[0]["workload"], "points": [p for t in tables for p in t["points"]]})
         if args.merge_pmc:
-            table = merge_pmc(table, args.merge_pmc)
+            table = merge_pmc(table, args.merge_pmc, args.pmc_budget_log2)
     elif args.merge_pmc:
         table = load(args.out)
         if table is None:
             raise SystemExit("no table at %s to merge into" % args.out)
-        table = merge_pmc(table, args.merge_pmc)
+        table = merge_pmc(table, args.merge_pmc, args.pmc_budget_log2)
     else:
         walks = [w for w in WALKS if args.walk in ("all", w[1])]
         table = pick(sweep(1 << args.budget_log2, walks, args.shape))

@@ -13,6 +13,7 @@
 // whose public keys the reference computes on the CPU (src/gpu/mod.rs:210-
 // 228). Every GPU hit is re-derived from scratch on the CPU before it counts.
 #include <fcntl.h>
+#include <sched.h>
 #include <sys/stat.h>
 #include <unistd.h>
 
@@ -23,6 +24,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -42,8 +44,6 @@ constexpr uint32_t kMaxHits = 4096;                     // src/gpu/mod.rs:23
 constexpr double kTargetLaunchS = 0.5;                  // src/gpu/mod.rs:26
 constexpr double kReportIntervalS = 10.0;               // src/main.rs:37
 constexpr uint64_t kLaneStride = 1ull << 40;            // key distance between GPU lanes, at most
-constexpr uint32_t kGpuBatch = 256;                     // keys per inversion (the autotuner's pick)
-constexpr uint32_t kGpuWavesPerCu = 16;
 
 const char* kUsage =
     "usage: lxmf-vanity [--gpu [--cpu]] <hex prefix>\n"
@@ -56,6 +56,30 @@ const char* kUsage =
     "  --gpu            search on the GPU: HIP on an AMD Instinct MI355X (gfx950)\n"
     "  --gpu --cpu      search on the GPU and the remaining CPU cores together\n"
     "  --gpu-self-test  check the GPU's results against the CPU";
+
+// The CPUs this process may use, as Rust's available_parallelism counts
+// them (src/main.rs:462): the affinity mask, capped by a cgroup CPU quota.
+size_t available_parallelism() {
+    size_t n = 0;
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof set, &set) == 0) n = (size_t)CPU_COUNT(&set);
+    if (n == 0) {
+        const unsigned hw = std::thread::hardware_concurrency();
+        n = hw ? hw : 1;
+    }
+    // cgroup v2: "max" or "<quota> <period>" in cpu.max
+    if (FILE* f = std::fopen("/sys/fs/cgroup/cpu.max", "r")) {
+        char quota[32] = {0};
+        unsigned long long period = 0;
+        if (std::fscanf(f, "%31s %llu", quota, &period) == 2 && std::strcmp(quota, "max") != 0 && period) {
+            const unsigned long long q = std::strtoull(quota, nullptr, 10) / period;
+            n = std::min<size_t>(n, std::max<unsigned long long>(q, 1));
+        }
+        std::fclose(f);
+    }
+    return n;
+}
 
 double seconds_since(Clock::time_point t0) {
     return std::chrono::duration<double>(Clock::now() - t0).count();
@@ -87,10 +111,9 @@ class GpuSearch {
   public:
     static std::unique_ptr<GpuSearch> open(std::string* err) {
         std::unique_ptr<GpuSearch> g(new GpuSearch());
+        // batch and waves_per_cu 0: the library's tuned single-search shape
         lxv_open_opts o{};
         o.size = sizeof o;
-        o.batch = kGpuBatch;
-        o.waves_per_cu = kGpuWavesPerCu;
         if (lxv_open(&o, &g->ctx_) != LXV_OK) {
             *err = lxv_last_error(nullptr);
             return nullptr;
@@ -412,8 +435,7 @@ int main(int argc, char** argv) {
     std::printf("Looking for an LXMF address starting %s.\n", lower(words[0]).c_str());
     std::fflush(stdout);
 
-    const unsigned hw = std::thread::hardware_concurrency();
-    const size_t cores = hw ? hw : 1;
+    const size_t cores = available_parallelism();
     // With the GPU, one core is left to drive it.
     const size_t cpu_threads = !use_gpu ? cores : (with_cpu ? std::max<size_t>(cores - 1, 1) : 0);
     const auto start = Clock::now();

@@ -106,10 +106,15 @@ uint32_t bit_length(uint64_t v);
 
 struct MultiParams;
 struct MultiInitEntry;
-// walk: LXV_WALK_PROJECTIVE or LXV_WALK_PAIRS (pool state of 4 or 2 field
-// elements per virtual lane).
-bool multi_supported(int layout, int batch, int walk);
-hipError_t launch_multi(int layout, int batch, int walk, const MultiParams& p, hipStream_t s);
+// The batched kernels (lxv_multi_kernels.hip). walk: LXV_WALK_PROJECTIVE or
+// LXV_WALK_PAIRS (pool state of 4 or 2 field elements per virtual lane);
+// waves: per SIMD the instantiation's registers are budgeted for (2 or 4;
+// the projective form is built for 4 only). multi_waves_per_cu: waves of 64
+// one CU holds at once (0: not built); the grid of p.phys_lanes / 64 waves
+// should not exceed CUs x that.
+bool multi_supported(int layout, int batch, int walk, int waves);
+int multi_waves_per_cu(int layout, int batch, int walk, int waves);
+hipError_t launch_multi(int layout, int batch, int walk, int waves, const MultiParams& p, hipStream_t s);
 // Sets up the virtual lanes of n admitted searches (entries in device memory)
 // in one launch; max_vlanes bounds every entry's lane count. The pair walk
 // takes each entry's anchor p0 and s_anchor = stride Q (host memory, passed

@@ -18,7 +18,6 @@
 #include "fe8.h"
 #include "lxv_kernels.h"
 #include "lxv_walk.h"
-#include "lxv_multi.h"
 #include "lxv_subwalk.h"
 
 namespace lxv {
@@ -277,43 +276,6 @@ __global__ __launch_bounds__(256) void copy_kernel(const uint4* src, uint4* dst,
         dst[i] = src[i];
 }
 
-template <class F, int B>
-__global__ __launch_bounds__(64, 4) void multi_walk_kernel(const MultiParams p) {
-    const uint32_t plane = blockIdx.x * 64u + threadIdx.x;
-    if (plane < p.phys_lanes) multi_wave<F, B>(p, plane);
-}
-
-template <class F, int B>
-__global__ __launch_bounds__(64, 4) void multi_pair_kernel(const MultiParams p) {
-    const uint32_t plane = blockIdx.x * 64u + threadIdx.x;
-    if (plane < p.phys_lanes) multi_pair_wave<F, B>(p, plane);
-}
-
-struct MultiInitArgs {
-    uint32_t q[8];
-    uint64_t stride;
-    uint32_t pool_lanes;
-    uint32_t bits;  // pair walk: every entry's vlanes - 1 < 2^bits
-    const MultiInitEntry* entries;
-    PairAnchor s_anchor;  // pair walk: stride Q
-};
-
-// Block (x, y): lanes 64 x .. 64 x + 63 of admitted search y.
-template <class F>
-__global__ __launch_bounds__(64) void multi_init_kernel(uint32_t* pool, const MultiInitArgs a) {
-    const MultiInitEntry& e = a.entries[blockIdx.y];
-    const uint32_t v = blockIdx.x * 64u + threadIdx.x;
-    if (v < e.vlanes) multi_init_lane<F>(pool, a.pool_lanes, e.state_off, e.base, a.stride, a.q, v);
-}
-
-template <class F>
-__global__ __launch_bounds__(64) void multi_pair_init_kernel(uint32_t* pool, const MultiInitArgs a) {
-    const MultiInitEntry& e = a.entries[blockIdx.y];
-    const uint32_t v = blockIdx.x * 64u + threadIdx.x;
-    if (v < e.vlanes)
-        multi_pair_init_lane<F>(pool, a.pool_lanes, e.state_off, e.p0, a.s_anchor, a.bits, v);
-}
-
 // ---------------------------------------------------------------------------
 // Launchers
 
@@ -450,64 +412,6 @@ hipError_t launch_init_sw_ladder(int layout, uint32_t* state, uint64_t* ns, uint
     const uint64_t total = (uint64_t)lanes * subwalks;
     hipLaunchKernelGGL(init_sw_ladder_kernel<Field8>, dim3((uint32_t)((total + 63) / 64)), dim3(64), 0, s,
                        reinterpret_cast<uint2*>(state), ns, xor_acc, a, subwalks);
-    return hipGetLastError();
-}
-
-typedef void (*multi_fn)(const MultiParams);
-struct MultiEntry {
-    int layout;
-    int batch;
-    int walk;
-    multi_fn fn;
-};
-static const MultiEntry kMultis[] = {
-    {LXV_LAYOUT_8X32, 64, LXV_WALK_PROJECTIVE, multi_walk_kernel<Field8, 64>},
-    {LXV_LAYOUT_8X32, 256, LXV_WALK_PROJECTIVE, multi_walk_kernel<Field8, 256>},
-    {LXV_LAYOUT_10X26, 256, LXV_WALK_PROJECTIVE, multi_walk_kernel<Field10, 256>},
-    {LXV_LAYOUT_8X32, 64, LXV_WALK_PAIRS, multi_pair_kernel<Field8, 64>},
-    {LXV_LAYOUT_8X32, 256, LXV_WALK_PAIRS, multi_pair_kernel<Field8, 256>},
-};
-
-bool multi_supported(int layout, int batch, int walk) {
-    for (const auto& e : kMultis)
-        if (e.layout == layout && e.batch == batch && e.walk == walk) return true;
-    return false;
-}
-
-hipError_t launch_multi(int layout, int batch, int walk, const MultiParams& p, hipStream_t s) {
-    for (const auto& e : kMultis) {
-        if (e.layout == layout && e.batch == batch && e.walk == walk) {
-            hipLaunchKernelGGL(e.fn, dim3((p.phys_lanes + 63) / 64), dim3(64), 0, s, p);
-            return hipGetLastError();
-        }
-    }
-    return hipErrorInvalidValue;
-}
-
-hipError_t launch_multi_init(int layout, int walk, uint32_t* pool, uint32_t pool_lanes, const uint32_t q[8],
-                             uint64_t stride, const MultiInitEntry* entries, uint32_t n, uint32_t max_vlanes,
-                             const PairAnchor* s_anchor, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    if (n > 65535 || max_vlanes == 0) return hipErrorInvalidValue;
-    MultiInitArgs a;
-    for (int i = 0; i < 8; i++) a.q[i] = q[i];
-    a.stride = stride;
-    a.pool_lanes = pool_lanes;
-    a.bits = bit_length(max_vlanes - 1);
-    a.entries = entries;
-    std::memset(&a.s_anchor, 0, sizeof a.s_anchor);
-    const dim3 grid((max_vlanes + 63) / 64, n), block(64);
-    if (walk == LXV_WALK_PAIRS && layout == LXV_LAYOUT_8X32) {
-        if (!s_anchor) return hipErrorInvalidValue;
-        a.s_anchor = *s_anchor;
-        hipLaunchKernelGGL(multi_pair_init_kernel<Field8>, grid, block, 0, s, pool, a);
-    } else if (walk == LXV_WALK_PAIRS) {
-        return hipErrorInvalidValue;
-    }
-    else if (layout == LXV_LAYOUT_8X32)
-        hipLaunchKernelGGL(multi_init_kernel<Field8>, grid, block, 0, s, pool, a);
-    else
-        hipLaunchKernelGGL(multi_init_kernel<Field10>, grid, block, 0, s, pool, a);
     return hipGetLastError();
 }
 

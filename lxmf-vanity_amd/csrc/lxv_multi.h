@@ -60,14 +60,14 @@ struct Unit {
     uint32_t nblocks;
 };
 
-struct MultiHit {
-    uint32_t slot;
-    uint32_t vlane;
-    uint64_t n;
-    uint32_t address[4];
-    uint32_t block;
-    uint32_t pad[3];
-};
+// A search's best hit of one step, as one 64-bit key ordered as walk order
+// within a block: key = n << kHitVlaneBits | v (n < 2^40 keys per virtual
+// lane, v < 2^22 virtual lanes), so the lowest key is the lowest (n, v); the
+// block is n / (R B). No key: kNoHit.
+constexpr int kHitVlaneBits = 22;
+constexpr unsigned long long kNoHit = ~0ull;
+
+LXV_HD unsigned long long hit_key(uint64_t n, uint32_t v) { return (n << kHitVlaneBits) | v; }
 
 struct MultiParams {
     uint32_t qp[10], qm[10];   // u(8B) +- 1 as raw limbs of the layout
@@ -75,49 +75,31 @@ struct MultiParams {
     SlotDyn* dyn;
     const Unit* __restrict__ units;
     const uint32_t* n_units;   // device word written with the unit list
-    uint32_t* counters;        // [0] next unit, [1] hit count, [2..3] keys (u64)
+    uint32_t* counters;        // [0] next unit, [2..3] keys (u64)
+    unsigned long long* best;  // [max slots]: this step's lowest hit key per slot (kNoHit: none)
     uint32_t* state;           // pool: [4 * kLimbs][pool_lanes] (pair walk: [2 * kLimbs][pool_lanes])
     uint2* batch_mem;          // per physical lane
-    MultiHit* hits;
     const uint32_t* stop;
     uint32_t pool_lanes;
     uint32_t phys_lanes;
-    uint32_t max_hits;
 };
 
 LXV_D uint32_t wave_uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
-// Hashes key n of virtual lane v (block blk) and records it if it matches:
-// one atomic per wave with matches, which also lowers the search's
-// lowest-hit block to blk.
+// Hashes key n of virtual lane v (block blk) and records it if it matches.
+// All lanes of a wave hash the same n (their v are consecutive), so the
+// wave's lowest matching (n, v) is its lowest matching lane: that lane alone
+// lowers the search's best key of this step and its lowest-hit block, with
+// two atomics per wave with matches and nothing stored per hit, so the work
+// of a step does not grow with its matches.
 LXV_D void multi_emit(const MultiParams& p, const SlotConst& sc, uint32_t slot, uint32_t v, uint32_t blk, uint64_t n,
                       const uint32_t uw[8]) {
-    const uint32_t me = __lane_id();
     uint32_t addr[4];
     lxmf_address_words(uw, sc.sha, addr);
-    const bool match = prefix_match(addr, sc.sha);
-    const uint64_t ball = __ballot(match);
-    if (ball) {
-        const uint32_t leader = (uint32_t)__builtin_ctzll(ball);
-        uint32_t base = 0;
-        if (me == leader) {
-            base = atomicAdd(&p.counters[1], (uint32_t)__builtin_popcountll(ball));
-            atomicMin(&p.dyn[slot].found_block, blk);
-        }
-        base = __shfl(base, (int)leader);
-        if (match) {
-            const uint32_t k = base + (uint32_t)__builtin_popcountll(ball & ((1ull << me) - 1ull));
-            if (k < p.max_hits) {
-                MultiHit h;
-                h.slot = slot;
-                h.vlane = v;
-                h.n = n;
-                for (int w = 0; w < 4; w++) h.address[w] = addr[w];
-                h.block = blk;
-                h.pad[0] = h.pad[1] = h.pad[2] = 0;
-                p.hits[k] = h;
-            }
-        }
+    const uint64_t ball = __ballot(prefix_match(addr, sc.sha));
+    if (ball && __lane_id() == (uint32_t)__builtin_ctzll(ball)) {
+        atomicMin(&p.best[slot], hit_key(n, v));
+        atomicMin(&p.dyn[slot].found_block, blk);
     }
 }
 

@@ -19,13 +19,13 @@
 #include "host_crypto.h"
 #include "lxv_kernels.h"
 #include "lxv_multi.h"
+#include "tuned_shapes.h"
 
 using namespace lxv;
 
 namespace {
 
 constexpr uint64_t kStride = 1ull << 40;   // key distance between virtual lanes
-constexpr uint32_t kMaxHits = 1u << 20;
 constexpr uint32_t kNone = 0xffffffffu;
 
 thread_local std::string g_err = "no error";
@@ -34,6 +34,7 @@ struct Search {
     uint64_t id = 0, tag = 0, max_keys = 0;
     uint32_t priority = 0;
     uint8_t start[32];
+    uint8_t ed_pub[32];
     SlotConst sc;
     uint32_t vlanes = 0, slot = kNone, state_off = 0;
     // Planned progress: once every launched step has run, blocks
@@ -49,7 +50,7 @@ struct Search {
     PairAnchor p0;  // the pair walk: (start / 8 + B/2 - 1) Q, from the host at submission
     // Best hit so far in walk order (block, n, v); final once its block is whole.
     bool hit = false;
-    uint32_t hit_block = 0, hit_vlane = 0, hit_addr[4] = {0, 0, 0, 0};
+    uint32_t hit_block = 0, hit_vlane = 0;
     uint64_t hit_n = 0;
     uint64_t keys = 0;
     double submit_s = 0, admit_s = 0;
@@ -130,7 +131,8 @@ struct lxv_multi {
     StepRec rec[2];
     PairAnchor s_anchor;    // the pair walk: stride Q, shared by every search
     std::string err = "no error";
-    int layout = LXV_LAYOUT_8X32, batch = 256, walk = LXV_WALK_PAIRS;
+    int layout = LXV_LAYOUT_8X32, batch = 256, walk = LXV_WALK_PAIRS, waves = 4;
+    double oldest_share = 0;  // of each step's work, for the longest-waiting resident search
     uint32_t rounds = 1, limbs = 8;
     uint32_t phys_lanes = 0, pool_lanes = 0, max_slots = 1024, max_vlanes = 1u << 18, lane_keys_log2 = 14;
     bool use_graph = true;
@@ -142,7 +144,7 @@ struct lxv_multi {
     uint32_t* d_counters = nullptr;  // 2 x 16 words, one block per parity
     uint32_t* d_pool = nullptr;
     uint2* d_batch = nullptr;
-    MultiHit* d_hits = nullptr;      // 2 x kMaxHits, one half per parity
+    unsigned long long* d_best = nullptr;  // 2 x max_slots: each step's lowest hit key per slot, one half per parity
     uint32_t* h_stop = nullptr;
     uint32_t* d_stop = nullptr;
     // pinned staging: slot tables mirrored on the host, admission entries
@@ -155,7 +157,7 @@ struct lxv_multi {
     Unit* h_units[2] = {nullptr, nullptr};
     uint32_t* h_nunits[2] = {nullptr, nullptr};
     uint32_t units_cap = 0;
-    MultiHit* h_hits = nullptr;
+    unsigned long long* h_best = nullptr;  // pinned, max_slots
     uint32_t* h_counters = nullptr;  // pinned, 16 words
     // graphs, one per parity
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -208,7 +210,7 @@ static void multi_free(lxv_multi* m) {
     (void)hipFree(m->d_counters);
     (void)hipFree(m->d_pool);
     (void)hipFree(m->d_batch);
-    (void)hipFree(m->d_hits);
+    (void)hipFree(m->d_best);
     (void)hipFree(m->d_init);
     if (m->h_slots) (void)hipHostFree(m->h_slots);
     if (m->h_dyn) (void)hipHostFree(m->h_dyn);
@@ -221,7 +223,7 @@ static void multi_free(lxv_multi* m) {
         if (m->ev0[p]) (void)hipEventDestroy(m->ev0[p]);
         if (m->ev1[p]) (void)hipEventDestroy(m->ev1[p]);
     }
-    if (m->h_hits) (void)hipHostFree(m->h_hits);
+    if (m->h_best) (void)hipHostFree(m->h_best);
     if (m->h_counters) (void)hipHostFree(m->h_counters);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     if (m->rstream) (void)hipStreamDestroy(m->rstream);
@@ -256,7 +258,15 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
     if (!m) return mfail(nullptr, LXV_E_NOMEM, "lxv_multi_open: out of host memory");
     m->t0 = std::chrono::steady_clock::now();
     m->device = o.device < 0 ? 0 : o.device;
-    m->batch = o.batch ? (int)o.batch : 256;
+    const lxv_tuned_shape& tuned = kTunedShapes[LXV_TUNED_BATCHED];
+    m->batch = o.batch ? (int)o.batch : (int)tuned.batch;
+    const uint32_t wpc = o.waves_per_cu ? o.waves_per_cu : tuned.waves_per_cu;
+    m->waves = wpc >= 16 ? 4 : (wpc >= 8 ? 2 : 1);
+    if (o.oldest_share_pct > 90) {
+        delete m;
+        return mfail(nullptr, LXV_E_ARG, "lxv_multi_open: oldest_share_pct must be 0..90");
+    }
+    m->oldest_share = o.oldest_share_pct / 100.0;
     m->layout = (int)o.layout;
     m->walk = (int)o.walk;
     m->use_graph = !o.no_graph;
@@ -270,9 +280,10 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
         }
         m->lane_keys_log2 = o.lane_keys_log2;
     }
-    if (!multi_supported(m->layout, m->batch, m->walk)) {
+    if (!multi_supported(m->layout, m->batch, m->walk, m->waves)) {
         delete m;
-        return mfail(nullptr, LXV_E_ARG, "lxv_multi_open: no batched kernel for this layout, batch and walk");
+        return mfail(nullptr, LXV_E_ARG,
+                     "lxv_multi_open: no batched kernel for this layout, batch, walk and waves per CU");
     }
     int ndev = 0;
     hipDeviceProp_t prop;
@@ -282,7 +293,13 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
         return mfail(nullptr, LXV_E_HIP, "no HIP device found (is this an MI355X host?)");
     }
     m->limbs = (uint32_t)limbs_of(m->layout);
-    m->phys_lanes = (uint32_t)prop.multiProcessorCount * 16u * 64u;
+    // the persistent grid: exactly the waves one round of the device holds
+    const int per_cu = multi_waves_per_cu(m->layout, m->batch, m->walk, m->waves);
+    if (per_cu <= 0) {
+        delete m;
+        return mfail(nullptr, LXV_E_HIP, "lxv_multi_open: cannot read the batched kernel's occupancy");
+    }
+    m->phys_lanes = (uint32_t)prop.multiProcessorCount * (uint32_t)per_cu * 64u;
     const uint64_t pool_bytes = o.pool_bytes ? o.pool_bytes : (2ull << 30);
     // pool state per virtual lane: the pair walk's centre (x, y), or the
     // projective walk's X(n-1), Z(n-1), X(n), Z(n)
@@ -317,7 +334,7 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
     MALLOC(m->d_pool, (size_t)m->pool_lanes * lane_bytes);
     MALLOC(m->d_batch, (size_t)m->phys_lanes * (m->walk == LXV_WALK_PAIRS ? m->batch / 2 : m->batch * 2) * m->limbs *
                            sizeof(uint32_t));
-    MALLOC(m->d_hits, 2 * (size_t)kMaxHits * sizeof(MultiHit));
+    MALLOC(m->d_best, 2 * (size_t)m->max_slots * sizeof(unsigned long long));
     MALLOC(m->d_init, (size_t)m->max_slots * sizeof(MultiInitEntry));
 #undef MALLOC
     hipError_t e = hipHostMalloc((void**)&m->h_stop, 64, hipHostMallocMapped | hipHostMallocCoherent);
@@ -327,7 +344,8 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
     if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_dyn, (size_t)m->max_slots * sizeof(SlotDyn), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_init, (size_t)m->max_slots * sizeof(MultiInitEntry), hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev_admit, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_hits, (size_t)kMaxHits * sizeof(MultiHit), hipHostMallocDefault);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&m->h_best, (size_t)m->max_slots * sizeof(unsigned long long), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_counters, 64, hipHostMallocDefault);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->rstream, hipStreamNonBlocking);
@@ -360,11 +378,10 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
         mp.counters = m->d_counters + 16 * p;
         mp.state = m->d_pool;
         mp.batch_mem = m->d_batch;
-        mp.hits = m->d_hits + (size_t)kMaxHits * p;
+        mp.best = m->d_best + (size_t)m->max_slots * p;
         mp.stop = m->d_stop;
         mp.pool_lanes = m->pool_lanes;
         mp.phys_lanes = m->phys_lanes;
-        mp.max_hits = kMaxHits;
     }
     *out = m;
     return LXV_OK;
@@ -385,6 +402,7 @@ extern "C" int lxv_multi_submit(lxv_multi* m, const lxv_search_spec* spec, uint6
     s.max_keys = spec->max_keys;
     s.priority = spec->priority;
     std::memcpy(s.start, spec->start, 32);
+    std::memcpy(s.ed_pub, spec->ed_pub, 32);
     s.vlanes = vlanes_for(p.nibbles, m->max_vlanes, m->lane_keys_log2);
     uint32_t ed[8], want[4], mask[4], nh[3];
     for (int i = 0; i < 8; i++) ed[i] = be_word(spec->ed_pub + 4 * i);
@@ -421,7 +439,7 @@ extern "C" int lxv_multi_submit(lxv_multi* m, const lxv_search_spec* spec, uint6
     s.sc.stride = kStride;
     s.sc.vlanes = s.vlanes;
     s.sc.rounds = m->rounds;
-    if (m->walk == LXV_WALK_PAIRS) pair_anchor<Field8>(s.sc.base, s.p0);
+    if (m->walk == LXV_WALK_PAIRS) pair_anchor<Field8>(s.sc.base, s.p0);  // canonical words: either layout
     s.submit_s = m->now();
     m->searches[s.id] = s;
     m->queue.push_back(s.id);
@@ -429,12 +447,22 @@ extern "C" int lxv_multi_submit(lxv_multi* m, const lxv_search_spec* spec, uint6
     return LXV_OK;
 }
 
-static void finish(lxv_multi* m, Search& s, int status, uint64_t g, const uint32_t addr[4]) {
+// The LXMF address of key start + 8 g, re-derived from scratch on the host
+// (X25519 and both hashes), as big-endian words.
+static void answer_address(const Search& s, uint64_t g, uint32_t out[4]) {
+    uint8_t k[32], u[32], a[16];
+    step_scalar(s.start, g, k);
+    x25519_base(k, u);
+    lxmf_address(u, s.ed_pub, a);
+    for (int i = 0; i < 4; i++) out[i] = be_word(a + 4 * i);
+}
+
+static void finish(lxv_multi* m, Search& s, int status, uint64_t g, bool found) {
     lxv_search_result r{};
     r.id = s.id;
     r.tag = s.tag;
     r.g = g;
-    if (addr) std::memcpy(r.address, addr, 16);
+    if (found) answer_address(s, g, r.address);
     r.keys = s.keys;
     r.vlanes = s.vlanes;
     r.status = status;
@@ -466,6 +494,11 @@ static int admit(lxv_multi* m) {
     std::stable_sort(m->queue.begin(), m->queue.end(), [m](uint64_t a, uint64_t b) {
         return m->searches.at(a).priority < m->searches.at(b).priority;
     });
+    if (m->oldest_share > 0) {
+        // the longest-waiting queued search goes first, whatever its priority
+        auto first = std::min_element(m->queue.begin(), m->queue.end());
+        std::rotate(m->queue.begin(), first, first + 1);
+    }
     uint32_t n = 0, max_vl = 0;
     for (auto qi = m->queue.begin(); qi != m->queue.end() && !m->free_slots.empty();) {
         Search& s = m->searches.at(*qi);
@@ -511,6 +544,20 @@ static int admit(lxv_multi* m) {
     return LXV_OK;
 }
 
+// Every slot's best hit key of the step with parity p set to kNoHit (all
+// ones) by the fill kernel, which a captured graph replays correctly.
+__global__ void fill_ones_kernel(uint32_t* words, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) words[i] = 0xffffffffu;
+}
+
+static hipError_t clear_best(lxv_multi* m, int p) {
+    const uint32_t n = 2 * m->max_slots;
+    hipLaunchKernelGGL(fill_ones_kernel, dim3((n + 255) / 256), dim3(256), 0, m->stream,
+                       reinterpret_cast<uint32_t*>(m->d_best + (size_t)m->max_slots * p), n);
+    return hipGetLastError();
+}
+
 static int launch_step(lxv_multi* m, int p) {
     uint32_t* counters = m->d_counters + 16 * p;
     if (m->use_graph) {
@@ -520,7 +567,8 @@ static int launch_step(lxv_multi* m, int p) {
             MCHK(m, hipStreamSynchronize(m->stream));
             MCHK(m, hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
             hipError_t e = launch_clear(counters, 16, m->stream);
-            if (e == hipSuccess) e = launch_multi(m->layout, m->batch, m->walk, m->params[p], m->stream);
+            if (e == hipSuccess) e = clear_best(m, p);
+            if (e == hipSuccess) e = launch_multi(m->layout, m->batch, m->walk, m->waves, m->params[p], m->stream);
             hipError_t e2 = hipStreamEndCapture(m->stream, &m->graph[p]);
             if (e != hipSuccess) return mfail(m, LXV_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
             MCHK(m, e2);
@@ -531,8 +579,9 @@ static int launch_step(lxv_multi* m, int p) {
         MCHK(m, hipEventRecord(m->ev1[p], m->stream));
     } else {
         MCHK(m, launch_clear(counters, 16, m->stream));
+        MCHK(m, clear_best(m, p));
         MCHK(m, hipEventRecord(m->ev0[p], m->stream));
-        MCHK(m, launch_multi(m->layout, m->batch, m->walk, m->params[p], m->stream));
+        MCHK(m, launch_multi(m->layout, m->batch, m->walk, m->waves, m->params[p], m->stream));
         MCHK(m, hipEventRecord(m->ev1[p], m->stream));
     }
     return LXV_OK;
@@ -563,25 +612,39 @@ static int plan_step(lxv_multi* m, uint32_t budget_ms, int p, bool* launched) {
         if (a.second->priority != b.second->priority) return a.second->priority < b.second->priority;
         return a.first != b.first ? a.first < b.first : a.second->id < b.second->id;
     });
+    // the longest-waiting resident search (lowest id: submission order)
+    // first gets oldest_share of the step, up to what its lanes can take
+    Search* oldest = nullptr;
+    if (m->oldest_share > 0)
+        for (Search* s : act)
+            if (!oldest || s->id < oldest->id) oldest = s;
     double left = m->rate * budget_s;
+    double reserved = 0;
+    if (oldest) {
+        for (const auto& w : want)
+            if (w.second == oldest) reserved = std::min(w.first, m->oldest_share * left);
+        left -= reserved;
+    }
     size_t n_units = 0;
     for (size_t i = 0; i < want.size(); i++) {
         Search* s = want[i].second;
         size_t tier_end = i;
         while (tier_end < want.size() && want[tier_end].second->priority == s->priority) tier_end++;
-        const double share = std::min(want[i].first, left / (double)(tier_end - i));
+        const double extra = s == oldest ? reserved : 0.0;
+        const double share = std::min(want[i].first - extra, left / (double)(tier_end - i));
         left -= share;
+        const double total = share + extra;
         const uint32_t vwaves = s->vlanes / 64;
         const double block_keys = wave_keys * vwaves;
-        if (share < 0.5 * wave_keys) {
+        if (total < 0.5 * wave_keys) {
             s->w0 = s->w1 = s->wave_cursor;  // nothing this step
             s->nblocks = 0;
-        } else if (s->wave_cursor == 0 && share >= block_keys && !s->hit) {
+        } else if (s->wave_cursor == 0 && total >= block_keys && !s->hit) {
             s->w0 = 0;
             s->w1 = vwaves;
-            s->nblocks = (uint32_t)std::floor(share / block_keys);
+            s->nblocks = (uint32_t)std::floor(total / block_keys);
         } else {
-            const uint32_t k = (uint32_t)std::max(1.0, std::round(share / wave_keys));
+            const uint32_t k = (uint32_t)std::max(1.0, std::round(total / wave_keys));
             s->w0 = s->wave_cursor;
             s->w1 = std::min(vwaves, s->wave_cursor + k);
             s->nblocks = 1;
@@ -666,42 +729,35 @@ static int collect_step(lxv_multi* m, int p) {
     // on the read-back stream, so a step enqueued since is not waited for
     MCHK(m, hipMemcpyAsync(m->h_counters, m->d_counters + 16 * p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
                            m->rstream));
+    MCHK(m, hipMemcpyAsync(m->h_best, m->d_best + (size_t)m->max_slots * p, m->max_slots * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, m->rstream));
     MCHK(m, hipStreamSynchronize(m->rstream));
     const uint64_t kkeys = (uint64_t)m->h_counters[2] | (uint64_t)m->h_counters[3] << 32;
-    const uint32_t nh = m->h_counters[1];
-    if (nh > kMaxHits) return mfail(m, LXV_E_STATE, "lxv_multi_step: hit buffer overflow");
-    if (nh) {
-        MCHK(m, hipMemcpyAsync(m->h_hits, m->d_hits + (size_t)kMaxHits * p, (size_t)nh * sizeof(MultiHit),
-                               hipMemcpyDeviceToHost, m->rstream));
-        MCHK(m, hipStreamSynchronize(m->rstream));
-    }
     m->steps++;
     m->keys += kkeys;
     m->busy_s += ms / 1e3;
     m->last_ms = ms;
     if (ms > 1.0 && kkeys > 0) m->rate = 0.5 * m->rate + 0.5 * (kkeys / (ms / 1e3));
-    // the searches this step walked, by slot, if they still hold it
-    std::vector<Search*> by_slot(m->max_slots, nullptr);
+    // the searches this step walked, if they still hold the slot they had
+    // when it was planned (searches finished since are skipped)
     std::vector<Search*> walked;
     for (const auto& mem : m->rec[p].members) {
         auto it = m->searches.find(mem.first);
         if (it == m->searches.end() || !it->second.active || it->second.slot != mem.second) continue;
-        by_slot[mem.second] = &it->second;
-        walked.push_back(&it->second);
-    }
-    // Per search, keep the lowest hit in walk order: block, then n, then v.
-    for (uint32_t i = 0; i < nh; i++) {
-        const MultiHit& h = m->h_hits[i];
-        if (h.slot >= m->max_slots) return mfail(m, LXV_E_STATE, "lxv_multi_step: hit for a slot out of range");
-        Search* s = by_slot[h.slot];
-        if (!s) continue;  // its search has finished since the step was planned
-        if (!s->hit || h.block < s->hit_block ||
-            (h.block == s->hit_block && (h.n < s->hit_n || (h.n == s->hit_n && h.vlane < s->hit_vlane)))) {
+        Search* s = &it->second;
+        walked.push_back(s);
+        // keep the lowest hit in walk order: block, then n, then v (the key
+        // orders by (n, v), and n determines the block)
+        const unsigned long long key = m->h_best[mem.second];
+        if (key == kNoHit) continue;
+        const uint64_t n = key >> kHitVlaneBits;
+        const uint32_t v = (uint32_t)(key & ((1u << kHitVlaneBits) - 1));
+        const uint32_t blk = (uint32_t)(n / ((uint64_t)m->rounds * m->batch));
+        if (!s->hit || hit_key(n, v) < hit_key(s->hit_n, s->hit_vlane)) {
             s->hit = true;
-            s->hit_block = h.block;
-            s->hit_n = h.n;
-            s->hit_vlane = h.vlane;
-            std::memcpy(s->hit_addr, h.address, sizeof s->hit_addr);
+            s->hit_block = blk;
+            s->hit_n = n;
+            s->hit_vlane = v;
         }
     }
     const double block_of = (double)m->rounds * m->batch;  // keys per lane per block
@@ -711,12 +767,12 @@ static int collect_step(lxv_multi* m, int p) {
         if (s->hit && s->done_block > s->hit_block) {
             // every block up to the hit's is walked: the answer is final
             s->keys = (uint64_t)(s->hit_block + 1) * s->vlanes * (uint64_t)block_of;
-            finish(m, *s, 1, (uint64_t)s->hit_vlane * kStride + s->hit_n, s->hit_addr);
+            finish(m, *s, 1, (uint64_t)s->hit_vlane * kStride + s->hit_n, true);
             continue;
         }
         s->keys = (uint64_t)s->done_block * s->vlanes * (uint64_t)block_of +
                   (uint64_t)s->done_cursor * 64 * (uint64_t)block_of;
-        if (s->max_keys && s->keys >= s->max_keys) finish(m, *s, 0, 0, nullptr);
+        if (s->max_keys && s->keys >= s->max_keys) finish(m, *s, 0, 0, false);
     }
     return LXV_OK;
 }
@@ -735,13 +791,15 @@ extern "C" int lxv_multi_step(lxv_multi* m, uint32_t budget_ms, lxv_search_resul
     }
     if (m->pipelined) {
         // the step launched last call ran meanwhile; collect it while this
-        // one runs
-        if (m->inflight) {
+        // one runs. The step just launched is tracked first, so a failed
+        // collection leaves the engine consistent.
+        const bool had = m->inflight;
+        m->inflight = launched;
+        if (launched) m->parity = 1 - p;
+        if (had) {
             rc = collect_step(m, 1 - p);
             if (rc) return rc;
         }
-        m->inflight = launched;
-        if (launched) m->parity = 1 - p;
     } else if (launched) {
         rc = collect_step(m, p);
         if (rc) return rc;
@@ -760,7 +818,7 @@ extern "C" int lxv_multi_cancel(lxv_multi* m, uint64_t id) {
     auto it = m->searches.find(id);
     if (it == m->searches.end()) return mfail(m, LXV_E_ARG, "lxv_multi_cancel: no such search");
     if (!it->second.active) m->queue.erase(std::find(m->queue.begin(), m->queue.end(), id));
-    finish(m, it->second, -1, 0, nullptr);
+    finish(m, it->second, -1, 0, false);
     return LXV_OK;
 }
 

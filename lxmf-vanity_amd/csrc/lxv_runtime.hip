@@ -18,6 +18,7 @@
 #include "lxv_kernels.h"
 #include "lxv_pairwalk.h"
 #include "lxv_walk.h"
+#include "tuned_shapes.h"
 
 using namespace lxv;
 
@@ -181,7 +182,8 @@ extern "C" int lxv_open(const lxv_open_opts* opts, lxv_ctx** out) {
         delete c;
         return fail(nullptr, LXV_E_ARG, "lxv_open: unknown layout");
     }
-    uint32_t wpc = o.waves_per_cu ? o.waves_per_cu : 16;
+    const lxv_tuned_shape& tuned = kTunedShapes[LXV_TUNED_SINGLE];
+    uint32_t wpc = o.waves_per_cu ? o.waves_per_cu : tuned.waves_per_cu;
     c->waves = wpc >= 16 ? 4 : (wpc >= 8 ? 2 : 1);
     c->subwalks = o.subwalks;
     if (c->subwalks) {
@@ -193,7 +195,7 @@ extern "C" int lxv_open(const lxv_open_opts* opts, lxv_ctx** out) {
         }
         c->batch = 1;
     } else {
-        c->batch = o.batch ? o.batch : 64;
+        c->batch = o.batch ? o.batch : tuned.batch;
         c->walk = o.walk;
         if (c->walk != LXV_WALK_PAIRS && c->walk != LXV_WALK_PROJECTIVE) {
             delete c;
@@ -451,6 +453,10 @@ extern "C" int lxv_poll(lxv_ctx* c, int wait, lxv_hit* hits, uint32_t cap, uint3
                         size_t dump_words) {
     if (!c) return LXV_E_ARG;
     if (!c->in_flight) return fail(c, LXV_E_STATE, "lxv_poll: nothing launched");
+    // checked before anything is collected, so a too-small buffer leaves the
+    // launch in flight and the call can be repeated with a larger one
+    const size_t dw = (size_t)c->last.dump_threads * c->batch * 8;
+    if (dw && dump && dump_words < dw) return fail(c, LXV_E_ARG, "lxv_poll: dump buffer too small");
     HIPCHK(c, hipSetDevice(c->device));
     if (!wait) {
         hipError_t q = hipEventQuery(c->ev1);
@@ -476,11 +482,7 @@ extern "C" int lxv_poll(lxv_ctx* c, int wait, lxv_hit* hits, uint32_t cap, uint3
     const uint32_t give = stored < cap ? stored : cap;
     if (n_hits) *n_hits = stored;
     if (give && hits) HIPCHK(c, hipMemcpy(hits, c->d_hits, (size_t)give * sizeof(lxv_hit), hipMemcpyDeviceToHost));
-    const size_t dw = (size_t)c->last.dump_threads * c->batch * 8;
-    if (dw && dump) {
-        if (dump_words < dw) return fail(c, LXV_E_ARG, "lxv_poll: dump buffer too small");
-        HIPCHK(c, hipMemcpy(dump, c->d_dump, dw * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    }
+    if (dw && dump) HIPCHK(c, hipMemcpy(dump, c->d_dump, dw * sizeof(uint32_t), hipMemcpyDeviceToHost));
     return LXV_OK;
 }
 
@@ -587,6 +589,12 @@ extern "C" int lxv_last_stats(lxv_ctx* c, lxv_launch_stats* out) {
 extern "C" void lxv_close(lxv_ctx* c) { free_ctx(c); }
 
 extern "C" const char* lxv_last_error(lxv_ctx* c) { return c ? c->err.c_str() : g_open_error.c_str(); }
+
+extern "C" int lxv_tuned(int workload, lxv_tuned_shape* out) {
+    if (!out || workload < LXV_TUNED_SINGLE || workload > LXV_TUNED_SERVING) return LXV_E_ARG;
+    *out = kTunedShapes[workload];
+    return LXV_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Host helpers of the search driver.

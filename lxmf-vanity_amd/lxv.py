@@ -66,7 +66,17 @@ class MultiOpts(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("device", ctypes.c_int32), ("max_searches", ctypes.c_uint32),
                 ("batch", ctypes.c_uint32), ("layout", ctypes.c_uint32), ("no_graph", ctypes.c_uint32),
                 ("pool_bytes", ctypes.c_uint64), ("max_vlanes", ctypes.c_uint32), ("lane_keys_log2", ctypes.c_uint32),
-                ("walk", ctypes.c_uint32), ("pipelined", ctypes.c_uint32)]
+                ("walk", ctypes.c_uint32), ("pipelined", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32),
+                ("oldest_share_pct", ctypes.c_uint32)]
+
+
+class TunedShape(ctypes.Structure):
+    _fields_ = [("batch", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32), ("layout", ctypes.c_uint32),
+                ("walk", ctypes.c_uint32), ("step_ms", ctypes.c_uint32), ("lane_keys_log2", ctypes.c_uint32),
+                ("keys_per_s", ctypes.c_double)]
+
+
+TUNED_WORKLOADS = {"single": 0, "batched": 1, "serving": 2}
 
 
 class SearchSpec(ctypes.Structure):
@@ -405,12 +415,16 @@ class Multi:
     keys, vlanes, status (1 found, 0 gave up, -1 cancelled), submit_s,
     admit_s, done_s (seconds since open). pipelined: each step() enqueues
     the next step before collecting the previous one, whose answers it
-    returns."""
+    returns. batch = 0 and waves_per_cu = 0 take the library's tuned shape
+    for batched searches; oldest_share_pct reserves that share of every
+    step for the longest-waiting search."""
 
-    def __init__(self, max_searches=0, batch=256, layout=LAYOUT_8X32, no_graph=False, pool_bytes=0,
-                 max_vlanes=0, lane_keys_log2=0, device=0, walk=WALK_PAIRS, pipelined=False):
+    def __init__(self, max_searches=0, batch=0, layout=LAYOUT_8X32, no_graph=False, pool_bytes=0,
+                 max_vlanes=0, lane_keys_log2=0, device=0, walk=WALK_PAIRS, pipelined=False, waves_per_cu=0,
+                 oldest_share_pct=0):
         o = MultiOpts(ctypes.sizeof(MultiOpts), device, max_searches, batch, layout, 1 if no_graph else 0,
-                      pool_bytes, max_vlanes, lane_keys_log2, walk, 1 if pipelined else 0)
+                      pool_bytes, max_vlanes, lane_keys_log2, walk, 1 if pipelined else 0, waves_per_cu,
+                      oldest_share_pct)
         h = ctypes.c_void_p()
         if lib().lxv_multi_open(ctypes.byref(o), ctypes.byref(h)) != LXV_OK:
             raise GpuUnavailable(lib().lxv_multi_last_error(None).decode())
@@ -475,6 +489,18 @@ def debug_fe(op, a_words, b_words, on_gpu, layout=LAYOUT_8X32):
     if rc != LXV_OK:
         raise RuntimeError("lxv_debug_fe failed (%d)" % rc)
     return list(O[:8 * n])
+
+
+def tuned_shape(workload):
+    """The launch shape the library was built with for `workload` ("single",
+    "batched" or "serving"), from the autotuner's table: a dict of batch,
+    waves_per_cu, layout, walk, step_ms, lane_keys_log2, keys_per_s."""
+    L = lib()
+    L.lxv_tuned.argtypes = [ctypes.c_int, ctypes.POINTER(TunedShape)]
+    t = TunedShape()
+    if L.lxv_tuned(TUNED_WORKLOADS[workload], ctypes.byref(t)) != LXV_OK:
+        raise ValueError("no tuned shape for %r" % workload)
+    return {k: getattr(t, k) for k, _ in TunedShape._fields_}
 
 
 def debug_valu_rate(op, waves_per_cu=16, device=0):

@@ -9,9 +9,15 @@ serving(): requests arrive as a Poisson process (rate 20/s by default) with
 prefix lengths 5-9 drawn with P = .30/.30/.20/.15/.05, each a fresh identity
 (its own start scalar and Ed25519 key); the loop admits them into the
 persistent kernel as slots and HBM pool memory allow, in a chosen order
-(shortest expected work first, or arrival order), and reports p50/p99
-time-to-hit from arrival, queue depth over time, aggregate keys/s and the
-requests still unfinished at the end. one_by_one() replays the same arrivals
+(shortest expected work first, optionally with a share of every step for the
+longest-waiting request, or arrival order), and reports p50/p99 time-to-hit
+from arrival (over finished requests, and over all with the unfinished
+counted at their age), queue depth over time, aggregate keys/s and the
+requests still unfinished at the end.
+
+Both run at the launch shape the library was tuned for (lxv.tuned_shape:
+keys per inversion, waves per CU, layout, step budget, keys per virtual
+lane), unless the caller overrides it. one_by_one() replays the same arrivals
 through the plain kernel, one search at a time in arrival order, as the
 reference's binary would serve them (src/gpu/mod.rs:146-174).
 
@@ -71,10 +77,26 @@ def _log(progress, msg):
         progress(msg)
 
 
-def batched(n=256, wall_s=60.0, budget_ms=200, seed=3, pool_bytes=8 << 30, lane_keys_log2=0, progress=print):
+def shape_for(workload, **override):
+    """The tuned launch shape of `workload` with the caller's overrides (keys
+    whose value is None or 0 keep the tuned value)."""
+    sh = lxv.tuned_shape(workload)
+    sh.update({k: v for k, v in override.items() if v})
+    return sh
+
+
+def open_multi(sh, **kw):
+    return lxv.Multi(batch=sh["batch"], waves_per_cu=sh["waves_per_cu"], layout=sh["layout"], walk=sh["walk"],
+                     lane_keys_log2=sh["lane_keys_log2"], **kw)
+
+
+def batched(n=256, wall_s=60.0, budget_ms=0, seed=3, pool_bytes=8 << 30, lane_keys_log2=0, progress=print,
+            shape=None):
+    sh = shape_for("batched", step_ms=budget_ms, lane_keys_log2=lane_keys_log2, **(shape or {}))
+    budget_ms = sh["step_ms"]
     rng = random.Random(seed)
     reqs = [Request(3_000_000 + i, draw_length(rng, BATCHED_LENGTHS)) for i in range(n)]
-    m = lxv.Multi(pool_bytes=pool_bytes, lane_keys_log2=lane_keys_log2, pipelined=True)
+    m = open_multi(sh, pool_bytes=pool_bytes, pipelined=True)
     try:
         by_id = {}
         for i, r in enumerate(reqs):
@@ -114,7 +136,7 @@ def batched(n=256, wall_s=60.0, budget_ms=200, seed=3, pool_bytes=8 << 30, lane_
             d["mean_keys_found"] += r.keys
     for d in per_len.values():
         d["mean_keys_found"] = d["mean_keys_found"] / d["found"] if d["found"] else None
-    return {"searches": n, "found": found, "rechecked": found, "wall_s": wall, "keys": st["keys"],
+    return {"searches": n, "found": found, "rechecked": found, "wall_s": wall, "keys": st["keys"], "shape": sh,
             "busy_s": st["busy_s"], "steps": st["steps"], "keys_per_s": st["keys"] / wall,
             "kernel_keys_per_s": st["keys"] / st["busy_s"] if st["busy_s"] else None,
             "per_length": {str(k): v for k, v in sorted(per_len.items())}}
@@ -153,22 +175,34 @@ def _latency_report(reqs, end):
             "per_length": table}
 
 
-def serving(rate=20.0, duration_s=60.0, policy="short-first", budget_ms=25, seed=4, lane_keys_log2=10,
-            pool_bytes=8 << 30, progress=print):
+POLICIES = ("short-first", "short-first-oldest", "fifo")
+
+
+def serving(rate=20.0, duration_s=60.0, policy="short-first-oldest", budget_ms=0, seed=4, lane_keys_log2=0,
+            pool_bytes=8 << 30, progress=print, oldest_share_pct=20, shape=None):
     """The serving loop. Requests are admitted into lxv_multi in `policy`
-    order: "short-first" (fewest expected keys, 16^digits, then arrival) or
-    "fifo". Each step walks about budget_ms of work over every resident
-    request; time-to-hit is from arrival to the answer being on the host.
-    Steps of 25 ms: over 60 s on MI355X, 15/25/35/50/100 ms steps gave p50
-    0.053/0.050/0.060/0.076/0.117 s at 87/92/94/92/95% of the plain
+    order: "short-first" (priority = digits: fewest expected keys, 16^digits,
+    then arrival; strict, so under overload the longest requests starve),
+    "short-first-oldest" (the same, but every step gives oldest_share_pct of
+    its work to the longest-waiting resident request, which is also admitted
+    first) or "fifo" (one priority: processor sharing in arrival order).
+    Each step walks about budget_ms of work over every resident request;
+    time-to-hit is from arrival to the answer being on the host. Measured on
+    MI355X over 60 s (8x32, B = 256, 16 waves/CU): 15/25/35/50/100 ms steps
+    gave p50 0.053/0.050/0.060/0.076/0.117 s at 87/92/94/92/95% of the plain
     kernel's keys/s."""
+    if policy not in POLICIES:
+        raise ValueError("policy must be one of %s" % (POLICIES,))
+    sh = shape_for("serving", step_ms=budget_ms, lane_keys_log2=lane_keys_log2, **(shape or {}))
+    budget_ms, lane_keys_log2 = sh["step_ms"], sh["lane_keys_log2"]
     trace = arrivals(rate, duration_s, seed, SERVING_LENGTHS)
     reqs = []
     # One step at a time: pipelining the steps keeps the GPU busier (measured
     # over 60 s: 93.1% of the plain kernel's keys/s against 91.7%) but makes a
     # new request wait for the step already in flight (p50 0.112 s against
     # 0.076 s), and time-to-hit is what this workload is for.
-    m = lxv.Multi(pool_bytes=pool_bytes, lane_keys_log2=lane_keys_log2)
+    m = open_multi(sh, pool_bytes=pool_bytes,
+                   oldest_share_pct=oldest_share_pct if policy == "short-first-oldest" else 0)
     try:
         by_id = {}
         t0 = time.perf_counter()
@@ -192,7 +226,7 @@ def serving(rate=20.0, duration_s=60.0, policy="short-first", budget_ms=25, seed
                 r = Request(s, digits, t_arr)
                 reqs.append(r)
                 # admission order inside the library: (priority, submission)
-                prio = digits if policy == "short-first" else 0
+                prio = 0 if policy == "fifo" else digits
                 by_id[m.submit(r.start, r.ed_pub, r.prefix, priority=prio)] = r
                 nxt += 1
             st = m.stats()
@@ -215,7 +249,8 @@ def serving(rate=20.0, duration_s=60.0, policy="short-first", budget_ms=25, seed
     finally:
         m.close()
     out = _latency_report(reqs, end)
-    out.update({"policy": policy, "rate_per_s": rate, "duration_s": duration_s, "wall_s": end,
+    out.update({"policy": policy, "oldest_share_pct": oldest_share_pct if policy == "short-first-oldest" else 0,
+                "shape": sh, "rate_per_s": rate, "duration_s": duration_s, "wall_s": end,
                 "budget_ms": budget_ms, "lane_keys_log2": lane_keys_log2,
                 "keys": st["keys"], "busy_s": st["busy_s"], "steps": st["steps"],
                 "keys_per_s": st["keys"] / end, "rechecked": out["finished"],
@@ -223,7 +258,7 @@ def serving(rate=20.0, duration_s=60.0, policy="short-first", budget_ms=25, seed
     return out
 
 
-def one_by_one(rate=20.0, duration_s=60.0, seed=4, batch=256, waves_per_cu=16, progress=print):
+def one_by_one(rate=20.0, duration_s=60.0, seed=4, batch=0, waves_per_cu=0, progress=print):
     """The same arrivals through the plain kernel, one search at a time in
     arrival order, every lane on the current search; launches grow while a
     launch is short (src/gpu/mod.rs:146-174). Every request sets up all
@@ -231,6 +266,8 @@ def one_by_one(rate=20.0, duration_s=60.0, seed=4, batch=256, waves_per_cu=16, p
     one thread per lane rather than the default 64 rounds on the persistent
     grid: an eighth of the set-up per request."""
     trace = arrivals(rate, duration_s, seed, SERVING_LENGTHS)
+    single = lxv.tuned_shape("single")
+    batch, waves_per_cu = batch or single["batch"], waves_per_cu or single["waves_per_cu"]
     probe = lxv.Backend(batch=batch, waves_per_cu=waves_per_cu)
     grid, lanes = probe.walk_kernel()[1], probe.threads
     probe.close()

@@ -39,7 +39,24 @@ def test_merge_and_pick(tmp_path):
         for c, v in (("GRBM_GUI_ACTIVE", 8e6), ("SQ_ACTIVE_INST_VALU", 0.25 * 1024 * 1e6 / 4),
                      ("SQ_INSTS_VALU", 123.0)):
             w.writerow({"Kernel_Name": name, "Grid_Size": str(1 << 18), "Counter_Name": c, "Counter_Value": v})
-    table = autotune.merge_pmc(table, str(path))
+    table = autotune.merge_pmc(table, str(path), budget_log2=28)
     merged = [p for p in table["points"] if "valu_busy_pct" in p]
     assert len(merged) == 1 and merged[0]["batch"] == 256
-    assert abs(merged[0]["valu_busy_pct"] - 25.0) < 1e-9 and merged[0]["valu_insts_per_wave_key"] == 123.0
+    assert abs(merged[0]["valu_busy_pct"] - 25.0) < 1e-9
+    # 2^24 lanes x 256 keys = 2^32 keys per iteration, above the 2^28 budget:
+    # the pass walked one warm-up iteration and two of one, 3 x 2^32 keys
+    assert autotune.pmc_pass_keys(merged[0], 28) == 3 << 32
+    assert merged[0]["valu_insts_per_key"] == 123.0 * 64 / (3 << 32)
+
+
+def test_mix_pick():
+    pts = [dict(batch=64, waves_per_cu=16, layout_id=0, walk_id=0, layout="8x32", step_ms=25, lane_keys_log2=10,
+                keys_per_s=8.0e9, p50_s_with_unfinished=0.9),
+           dict(batch=256, waves_per_cu=16, layout_id=0, walk_id=0, layout="8x32", step_ms=25, lane_keys_log2=10,
+                keys_per_s=9.0e9, p50_s_with_unfinished=1.5),
+           dict(batch=128, waves_per_cu=8, layout_id=0, walk_id=0, layout="8x32", step_ms=15, lane_keys_log2=10,
+                keys_per_s=8.5e9, p50_s_with_unfinished=1.0)]
+    assert autotune.mix_pick("batched", pts)["batch"] == 256
+    # serving: lowest p50 among points within 10% of the best rate (8.0e9 is
+    # out, 8.5e9 is in)
+    assert autotune.mix_pick("serving", pts)["batch"] == 128

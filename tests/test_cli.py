@@ -66,6 +66,31 @@ def test_cpu_search_saves_a_private_identity(tmp_path):
     assert re.fullmatch(r"About \d+ tries in \d+\.\d s \(\d+/s\)\. Saved to a1b\.identity", lines[-1])
 
 
+def test_progress_line_every_ten_seconds_parses_as_fleet_sh_reads_it(tmp_path):
+    # An unreachable prefix (16 hex digits) on the CPU for about 11 s: the
+    # 10-second report (src/main.rs:407-421) must be on stdout by then, in
+    # the form fleet.sh reads (fleet.sh:100-103): the last line containing
+    # "tries, ", its first word the tries, and after "tries, " the rate up to
+    # "/s", both integers.
+    import signal
+    import time
+    p = subprocess.Popen([CLI, "0123456789abcdef"], cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                         text=True)
+    try:
+        time.sleep(11.5)
+    finally:
+        p.send_signal(signal.SIGTERM)
+        out, err = p.communicate(timeout=30)
+    lines = [ln for ln in out.splitlines() if "tries, " in ln]
+    assert len(lines) == 1, out
+    line = lines[-1]
+    assert re.fullmatch(r"\d+ tries, \d+/s; the average is about \d+ min away", line), line
+    tried = int(line.split(" ", 1)[0])             # ${line%% *}
+    rate = int(line.split("tries, ", 1)[1].split("/s", 1)[0])   # r=${line#*tries, }; ${r%%/s*}
+    assert tried > 0 and rate > 0 and 9 * rate <= tried <= 11 * rate
+    assert not os.path.exists(os.path.join(tmp_path, "0123456789abcdef.identity"))
+
+
 def test_an_existing_identity_file_is_never_overwritten(tmp_path):
     p = os.path.join(tmp_path, "c0.identity")
     with open(p, "wb") as f:

@@ -8,6 +8,9 @@ one step at a time or pipelined (the next step enqueued before the previous
 one is collected).
 """
 
+import random
+import time
+
 import pytest
 
 import oracle_lib as orc
@@ -17,6 +20,7 @@ pytestmark = pytest.mark.gpu
 
 STRIDE = lxv.STRIDE_MULTI
 B = 256
+CONFIG3_LENGTHS = ((4, .10), (5, .15), (6, .20), (7, .20), (8, .15), (9, .12), (10, .08))
 
 
 def workload(seeds, digits):
@@ -28,6 +32,8 @@ def workload(seeds, digits):
 
 
 def run(searches, budget_ms=50, priorities=None, **opts):
+    opts.setdefault("batch", B)
+    opts.setdefault("waves_per_cu", 16)
     m = lxv.Multi(**opts)
     try:
         prio = priorities or [0] * len(searches)
@@ -40,72 +46,158 @@ def run(searches, budget_ms=50, priorities=None, **opts):
         m.close()
 
 
-def oracle_first_hit(start, ed_pub, prefix, vlanes, upto_block):
-    ranges = [(v * STRIDE, (upto_block + 1) * B) for v in range(vlanes)]
+def oracle_first_hit(start, ed_pub, prefix, vlanes, upto_block, batch=B):
+    ranges = [(v * STRIDE, (upto_block + 1) * batch) for v in range(vlanes)]
     hits, _ = orc.scan_ranges(start, ed_pub, ranges, prefix, cap=1 << 16)
-    best = min(((g % STRIDE) // B, g % STRIDE, g // STRIDE, g, a) for g, a in hits)
+    best = min(((g % STRIDE) // batch, g % STRIDE, g // STRIDE, g, a) for g, a in hits)
     return best[3], best[4]
 
 
-@pytest.mark.parametrize("walk,pipelined", [(lxv.WALK_PAIRS, False), (lxv.WALK_PROJECTIVE, False),
-                                             (lxv.WALK_PAIRS, True)], ids=["pairs", "projective", "pairs-pipelined"])
-def test_answers_are_first_hits_in_walk_order(walk, pipelined):
+def check_first_hit(r, start, ed_pub, prefix, batch=B):
+    """r is the search's first hit in walk order, as the oracle walks it."""
+    g, addr = r["g"], r["address"]
+    assert addr == orc.lxmf_address(orc.x25519(orc.step_scalar(start, g)), ed_pub)
+    assert addr.hex().startswith(prefix)
+    block = (g % STRIDE) // batch
+    assert (g, addr) == oracle_first_hit(start, ed_pub, prefix, r["vlanes"], block, batch)
+    assert r["keys"] == (block + 1) * r["vlanes"] * batch
+
+
+# Every batched kernel the library builds: the pair walk at B = 64, 128, 256
+# (8 x 32) and 256 (10 x 25.5), each for 2 and 4 waves per SIMD (8 and 16
+# waves per CU); the projective walk at B = 64, 256 (8 x 32) and 256
+# (10 x 25.5); and the pair walk with pipelined steps.
+MULTI_KERNELS = [dict(walk=lxv.WALK_PAIRS, layout=lxv.LAYOUT_8X32, batch=b, waves_per_cu=w)
+                 for b in (64, 128, 256) for w in (8, 16)]
+MULTI_KERNELS += [dict(walk=lxv.WALK_PAIRS, layout=lxv.LAYOUT_10X26, batch=256, waves_per_cu=w) for w in (8, 16)]
+MULTI_KERNELS += [dict(walk=lxv.WALK_PROJECTIVE, layout=lxv.LAYOUT_8X32, batch=b, waves_per_cu=16) for b in (64, 256)]
+MULTI_KERNELS += [dict(walk=lxv.WALK_PROJECTIVE, layout=lxv.LAYOUT_10X26, batch=256, waves_per_cu=16)]
+MULTI_KERNELS += [dict(walk=lxv.WALK_PAIRS, layout=lxv.LAYOUT_8X32, batch=256, waves_per_cu=16, pipelined=True)]
+
+
+def kernel_id(k):
+    return "%s-%s-b%d-w%d%s" % ("pairs" if k["walk"] == lxv.WALK_PAIRS else "proj",
+                                "8x32" if k["layout"] == lxv.LAYOUT_8X32 else "10x26", k["batch"],
+                                k["waves_per_cu"], "-pipelined" if k.get("pipelined") else "")
+
+
+@pytest.mark.parametrize("kernel", MULTI_KERNELS, ids=[kernel_id(k) for k in MULTI_KERNELS])
+def test_answers_are_first_hits_in_walk_order(kernel):
     searches = workload(range(100, 112), [2, 3] * 6)
-    res, st = run(searches, walk=walk, pipelined=pipelined)
+    res, st = run(searches, **kernel)
     for i, (start, ed_pub, prefix) in enumerate(searches):
-        r = res[i]
-        assert r["status"] == 1
-        g, addr = r["g"], r["address"]
-        assert addr == orc.lxmf_address(orc.x25519(orc.step_scalar(start, g)), ed_pub)
-        assert addr.hex().startswith(prefix)
-        block = (g % STRIDE) // B
-        want_g, want_addr = oracle_first_hit(start, ed_pub, prefix, r["vlanes"], block)
-        assert (g, addr) == (want_g, want_addr), i
+        assert res[i]["status"] == 1
+        check_first_hit(res[i], start, ed_pub, prefix, kernel["batch"])
     assert st["done"] == len(searches) and st["keys"] > 0
 
 
-def test_same_answer_alone_and_in_a_mix():
-    targets = workload([200, 201, 202], [3, 4, 3])
-    others = workload(range(300, 320), [2, 3, 4, 5] * 5)
-    alone = [run([t])[0][0] for t in targets]
-    mixed, _ = run(others[:10] + targets + others[10:])
-    # priorities change who is admitted and served first, never an answer
-    prio = [i % 4 for i in range(10)] + [5, 5, 5] + [i % 3 for i in range(10)]
-    ranked, _ = run(others[:10] + targets + others[10:], budget_ms=5, priorities=prio)
-    piped, _ = run(others[:10] + targets + others[10:], budget_ms=5, priorities=prio, pipelined=True)
-    for i, a in enumerate(alone):
-        for res in (mixed, ranked, piped):
-            m = res[10 + i]
-            assert (m["g"], m["address"]) == (a["g"], a["address"])
+def test_config3_shaped_mix_of_256_searches():
+    # The batched workload's shape: 256 seeded searches with prefix lengths
+    # drawn from its 4-10 digit table, each with its own start and Ed25519
+    # key, in one persistent kernel. Searches give up after about 2^24 keys,
+    # so the 4-6 digit ones mostly finish and the longer ones give way; every
+    # finished answer must be the oracle's first hit in walk order.
+    rng = random.Random(3)
+    digits = []
+    for _ in range(256):
+        x, acc = rng.random(), 0.0
+        for d, pr in CONFIG3_LENGTHS:
+            acc += pr
+            if x < acc:
+                break
+        digits.append(d)
+    searches = workload(range(5_000_000, 5_000_256), digits)
+    m = lxv.Multi(batch=B, waves_per_cu=16, pool_bytes=4 << 30, pipelined=True)
+    try:
+        for i, (st, ed, pf) in enumerate(searches):
+            m.submit(st, ed, pf, tag=i, max_keys=1 << 24)
+        res = {r["tag"]: r for r in m.run_all(budget_ms=100, timeout_s=300)}
+    finally:
+        m.close()
+    assert sorted(res) == list(range(256))
+    found = [i for i in res if res[i]["status"] == 1]
+    print("found", len(found), "of 256; by length",
+          {d: sum(1 for i in found if digits[i] == d) for d in range(4, 11)}, flush=True)
+    assert sum(1 for i in found if digits[i] == 4) == digits.count(4)
+    assert len(found) >= digits.count(4) + digits.count(5) // 2
+    for i in found:
+        check_first_hit(res[i], *searches[i])
+    for i, r in res.items():
+        if r["status"] == 0:
+            assert r["keys"] >= 1 << 24
 
 
-def test_blocks_split_across_steps_keep_the_answer():
-    # 6 digits at 2^8 expected keys per lane: 65536 virtual lanes, so a block
-    # is 2^24 keys, far above a 1 ms step's share: each block is walked a few
-    # virtual waves per step. The answer must match whole-block steps and the
-    # oracle, and `keys` must count every block up to the answer's.
-    searches = workload(range(700, 704), [6] * 4)
-    split, st = run(searches, budget_ms=1, lane_keys_log2=8)
-    piped, st_p = run(searches, budget_ms=1, lane_keys_log2=8, pipelined=True)
-    whole, _ = run(searches, budget_ms=2000, lane_keys_log2=8)
-    assert st["steps"] > 8 and st_p["steps"] > 8
-    for i, (start, ed_pub, prefix) in enumerate(searches):
-        r = split[i]
-        assert r["vlanes"] == 65536 and r["status"] == 1
-        assert (r["g"], r["address"]) == (whole[i]["g"], whole[i]["address"])
-        assert (piped[i]["g"], piped[i]["address"], piped[i]["keys"]) == (r["g"], r["address"], r["keys"])
-        block = (r["g"] % STRIDE) // B
-        assert r["keys"] == (block + 1) * r["vlanes"] * B
-        assert (r["g"], r["address"]) == oracle_first_hit(start, ed_pub, prefix, r["vlanes"], block)
+def test_serving_admits_arrivals_while_others_run():
+    # Requests arrive as a Poisson process (60/s for 2 s, 3-5 hex digits:
+    # each needs several steps of 5 ms) and are submitted between steps,
+    # while earlier ones are resident on the GPU. Each answer must be the
+    # oracle's first hit in walk order and the same answer the request gets
+    # alone.
+    rng = random.Random(11)
+    t, trace = 0.0, []
+    while True:
+        t += rng.expovariate(60.0)
+        if t >= 2.0:
+            break
+        trace.append((t, 6_000_000 + len(trace), rng.choice((3, 4, 4, 5))))
+    m = lxv.Multi(batch=B, waves_per_cu=16, lane_keys_log2=10, oldest_share_pct=20)
+    reqs, res, overlapped = {}, {}, 0
+    try:
+        t0, nxt = time.perf_counter(), 0
+        while nxt < len(trace) or m.stats()["resident"] or m.stats()["queued"]:
+            now = time.perf_counter() - t0
+            while nxt < len(trace) and trace[nxt][0] <= now:
+                _, seed, d = trace[nxt]
+                (st, ed, pf), = workload([seed], [d])
+                if m.stats()["resident"]:
+                    overlapped += 1
+                sid = m.submit(st, ed, pf, tag=nxt, priority=d)
+                reqs[sid] = (st, ed, pf)
+                nxt += 1
+            for r in m.step(5):
+                res[r["id"]] = r
+            if time.perf_counter() - t0 > 120:
+                raise TimeoutError("serving loop did not drain")
+    finally:
+        m.close()
+    print("requests", len(trace), "submitted while others were resident", overlapped, flush=True)
+    assert len(res) == len(trace) and overlapped >= len(trace) // 4
+    for sid, r in res.items():
+        assert r["status"] == 1
+        check_first_hit(r, *reqs[sid])
+    # the same answer alone, for a sample of the requests
+    for sid in sorted(res)[::4]:
+        alone = run([reqs[sid]], budget_ms=5, lane_keys_log2=10)[0][0]
+        assert (alone["g"], alone["address"]) == (res[sid]["g"], res[sid]["address"])
 
 
-def test_graph_replay_matches_eager_launches():
-    searches = workload(range(400, 416), [3, 4] * 8)
-    g_res, g_st = run(searches)
-    e_res, e_st = run(searches, no_graph=True)
-    assert g_st["steps"] > 1
-    for i in range(len(searches)):
-        assert (g_res[i]["g"], g_res[i]["address"]) == (e_res[i]["g"], e_res[i]["address"])
+def test_oldest_share_bounds_the_wait_of_a_low_priority_search():
+    # A 4-digit search at priority 9, then a 10-digit search at priority 0
+    # whose lanes can take the whole GPU every step. With strict priority the
+    # first gets no work for as long as the second runs (40 steps of 5 ms
+    # here); with a 30% share for the longest-waiting search it finishes
+    # within them (its first hit is in its 13th block of 64 x 256 keys), with
+    # the answer the oracle gives.
+    (slow, hog) = workload([7_000_000, 7_000_001], [4, 10])
+
+    def serve(share):
+        m = lxv.Multi(batch=B, waves_per_cu=16, lane_keys_log2=10, oldest_share_pct=share)
+        try:
+            sid = m.submit(*slow, priority=9)
+            m.submit(*hog, priority=0, max_keys=1 << 40)
+            for step in range(40):
+                for r in m.step(5):
+                    if r["id"] == sid:
+                        return step, r
+            return None
+        finally:
+            m.close()
+
+    assert serve(0) is None
+    got = serve(30)
+    print("finished at step", got and got[0], flush=True)
+    assert got is not None
+    check_first_hit(got[1], *slow)
 
 
 @pytest.mark.parametrize("walk,lane_bytes,pipelined", [(lxv.WALK_PAIRS, 64, False), (lxv.WALK_PROJECTIVE, 128, False),
@@ -115,7 +207,7 @@ def test_admission_is_bounded_by_device_memory(walk, lane_bytes, pipelined):
     # room for three 64-lane searches (8 x 32-bit limbs: 2 or 4 field
     # elements of state per lane); finished searches hand their slots and
     # lanes to queued ones, also while a pipelined step is still running
-    m = lxv.Multi(pool_bytes=3 * 64 * lane_bytes, walk=walk, pipelined=pipelined)
+    m = lxv.Multi(batch=B, waves_per_cu=16, pool_bytes=3 * 64 * lane_bytes, walk=walk, pipelined=pipelined)
     try:
         searches = workload(range(500, 510), [3] * 10)
         for i, (st, ed, pf) in enumerate(searches):
@@ -139,7 +231,7 @@ def test_admission_is_bounded_by_device_memory(walk, lane_bytes, pipelined):
 def test_give_up_and_cancel(pipelined):
     # b is cancelled after its first step was launched (pipelined: while it
     # still runs); a gives up after about 2^22 keys
-    m = lxv.Multi(pipelined=pipelined)
+    m = lxv.Multi(batch=B, waves_per_cu=16, pipelined=pipelined)
     try:
         (s1, e1, _), (s2, e2, _) = workload([600, 601], [0, 0])
         a = m.submit(s1, e1, "0123456789", max_keys=1 << 22)

@@ -402,6 +402,29 @@ def test_valu_probe_rates_are_sane(op):
     assert 0 < rate < 2.0 * cus * 64 * clk
 
 
+def test_poll_with_a_small_dump_buffer_keeps_the_launch():
+    # lxv_poll refuses a dump buffer that is too small before collecting
+    # anything: the launch stays in flight and a second poll gets everything
+    import ctypes
+    b = lxv.Backend(threads=1024, batch=64)
+    try:
+        start, _, ed_pub, prefix = seeded(62, 2)
+        b.init_seeded(start, STRIDE)
+        b.launch_async(lxv.launch_args_for(ed_pub, prefix, iters=1, dump_threads=4, max_hits=1 << 12))
+        L = lxv.lib()
+        small = (ctypes.c_uint32 * 8)()
+        n = ctypes.c_uint32(0)
+        hits = (lxv.Hit * 4096)()
+        assert L.lxv_poll(b.h, 1, hits, 4096, ctypes.byref(n), small, 8) == -1
+        assert "dump buffer too small" in L.lxv_last_error(b.h).decode()
+        got, dump = b.poll(True, dump_threads=4, cap=4096)
+        want, _ = orc.scan_ranges(start, ed_pub, [(t * STRIDE, 64) for t in range(1024)], prefix, cap=1 << 12)
+        assert sorted((t * STRIDE + n_, a) for t, n_, a in got) == want
+        assert lxv.words_to_bytes_le(dump[:8]) == orc.x25519(start)
+    finally:
+        b.close()
+
+
 def test_self_test_and_run():
     start, ed_seed, _, _ = seeded(52, 0)
     s = lxv.GpuSearch(start=start, ed_seed=ed_seed, threads=16384)

@@ -55,7 +55,49 @@ def test_multi_options_are_checked_before_the_device():
     with pytest.raises(lxv.GpuUnavailable, match="lane_keys_log2"):
         lxv.Multi(lane_keys_log2=30)
     with pytest.raises(lxv.GpuUnavailable, match="no batched kernel"):
-        lxv.Multi(batch=128)
+        lxv.Multi(batch=32)
+    with pytest.raises(lxv.GpuUnavailable, match="no batched kernel"):
+        lxv.Multi(batch=256, waves_per_cu=4)
+    with pytest.raises(lxv.GpuUnavailable, match="no batched kernel"):
+        lxv.Multi(batch=128, layout=lxv.LAYOUT_10X26)
+    with pytest.raises(lxv.GpuUnavailable, match="oldest_share_pct"):
+        lxv.Multi(oldest_share_pct=95)
+
+
+def test_tuned_shapes_are_the_tables_picks():
+    # the library is built with the autotuner's picks from tuned.json
+    import json
+    with open(os.path.join(ROOT, "lxmf-vanity_amd", "tuned.json")) as f:
+        table = json.load(f)
+    import gen_tuned
+    picks = gen_tuned.picks(table)
+    for w in ("single", "batched", "serving"):
+        got = lxv.tuned_shape(w)
+        for k, j in (("batch", "batch"), ("waves_per_cu", "waves_per_cu"), ("layout", "layout_id"),
+                     ("walk", "walk_id"), ("step_ms", "step_ms"), ("lane_keys_log2", "lane_keys_log2")):
+            assert got[k] == picks[w][j], (w, k)
+
+
+def test_tuned_header_follows_the_table(tmp_path):
+    # a changed table gives a changed header (the build regenerates it)
+    import json
+    import gen_tuned
+    table = {"pick": {"batch": 64, "waves_per_cu": 8, "layout_id": 1, "walk_id": 0, "keys_per_s": 1.0},
+             "picks": {"serving": {"batch": 128, "waves_per_cu": 8, "layout_id": 0, "walk_id": 0, "step_ms": 15,
+                                   "lane_keys_log2": 9, "keys_per_s": 2.0}}}
+    src, dst = tmp_path / "t.json", tmp_path / "t.h"
+    src.write_text(json.dumps(table))
+    import sys
+    old = sys.argv
+    sys.argv = ["gen_tuned.py", str(src), str(dst)]
+    try:
+        gen_tuned.main()
+    finally:
+        sys.argv = old
+    text = dst.read_text()
+    assert "/* single */ {64, 8, 1, 0, 0, 0, 1.000000e+00}" in text
+    assert "/* serving */ {128, 8, 0, 0, 15, 9, 2.000000e+00}" in text
+    assert "/* batched */ {256, 16, 0, 0, 200, 14," in text
 
 
 def test_host_crypto_known_answers():
