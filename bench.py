@@ -372,7 +372,7 @@ def main():
     # default shape: 64 iterations of 2^30 keys); steps get floor or ceil
     total_iters = max(args.steps, budget // per_iter)
     step_iters = [total_iters // args.steps + (1 if i < total_iters % args.steps else 0) for i in range(args.steps)]
-    iters = step_iters[0]
+    iters = min(step_iters)   # warm-up launches: the common size, so profiles average close to the timed ones
     mean_iters = total_iters / args.steps
     a = lxv.launch_args_for(ed_pub, prefix, iters=iters, max_hits=lxv.MAX_HITS)
 

@@ -92,7 +92,9 @@ def open_multi(sh, **kw):
 
 def batched(n=256, wall_s=60.0, budget_ms=0, seed=3, pool_bytes=8 << 30, lane_keys_log2=0, progress=print,
             shape=None):
-    sh = shape_for("batched", step_ms=budget_ms, lane_keys_log2=lane_keys_log2, **(shape or {}))
+    ov = {"step_ms": budget_ms, "lane_keys_log2": lane_keys_log2}
+    ov.update(shape or {})
+    sh = shape_for("batched", **ov)
     budget_ms = sh["step_ms"]
     rng = random.Random(seed)
     reqs = [Request(3_000_000 + i, draw_length(rng, BATCHED_LENGTHS)) for i in range(n)]
@@ -193,7 +195,9 @@ def serving(rate=20.0, duration_s=60.0, policy="short-first-oldest", budget_ms=0
     kernel's keys/s."""
     if policy not in POLICIES:
         raise ValueError("policy must be one of %s" % (POLICIES,))
-    sh = shape_for("serving", step_ms=budget_ms, lane_keys_log2=lane_keys_log2, **(shape or {}))
+    ov = {"step_ms": budget_ms, "lane_keys_log2": lane_keys_log2}
+    ov.update(shape or {})
+    sh = shape_for("serving", **ov)
     budget_ms, lane_keys_log2 = sh["step_ms"], sh["lane_keys_log2"]
     trace = arrivals(rate, duration_s, seed, SERVING_LENGTHS)
     reqs = []
