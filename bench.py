@@ -60,14 +60,16 @@ def pair_mads_per_key(batch):
 
 
 def measure_mad_peak():
-    import ctypes
-    rate, clk, cus = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int(0)
-    L = lxv.lib()
-    L.lxv_debug_mad_rate.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
-                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
-    if L.lxv_debug_mad_rate(0, ctypes.byref(rate), ctypes.byref(clk), ctypes.byref(cus)) != 0:
-        return None, None, None
-    return rate.value, clk.value, cus.value
+    """The chip's v_mad_u64_u32 issue peak: the mad probe (8 independent
+    chains per lane, unrolled, lxv_debug_valu_rate) at 16 and 32 waves per
+    CU, the higher rate in lane-level mads/s; with the clock the probe
+    measured and the CU count."""
+    best = None
+    for wpc in (16, 32):
+        rate, clk, _, cus = lxv.debug_valu_rate(7, wpc)
+        if best is None or rate > best[0]:
+            best = (rate, clk, cus, wpc)
+    return best
 
 
 def cpu_model():
@@ -243,9 +245,10 @@ def serving_pass(args, plain_keys_per_s):
     import workloads
     log = lambda msg: print(msg, file=sys.stderr, flush=True)  # noqa: E731
     r = workloads.serving(duration_s=args.serving_s, policy=args.policy, oldest_share_pct=args.oldest_share_pct,
-                          progress=log)
+                          preempt=not args.no_preempt, progress=log)
     keep = ("requests", "finished", "unfinished", "p50_s", "p99_s", "p50_s_with_unfinished", "p99_s_with_unfinished",
-            "policy", "oldest_share_pct", "shape", "keys_per_s", "busy_s", "steps", "rechecked", "per_length")
+            "policy", "oldest_share_pct", "preempt", "shape", "keys_per_s", "busy_s", "steps", "rechecked",
+            "per_length")
     out = {k: r[k] for k in keep}
     out["workload"] = ("serving: Poisson 20/s for %.0f s, 5-9 hex digits (P=.30/.30/.20/.15/.05), a fresh seeded "
                        "identity per request, admission %s" % (args.serving_s, args.policy))
@@ -296,14 +299,15 @@ def many_search_main(args):
         if args.budget_ms:
             kw["budget_ms"] = args.budget_ms
         r = workloads.serving(duration_s=args.duration_s, policy=args.policy, progress=log,
-                              oldest_share_pct=args.oldest_share_pct, **kw)
+                              oldest_share_pct=args.oldest_share_pct, preempt=not args.no_preempt, **kw)
         out = {"metric": "p50/p99 time-to-hit, mixed prefixes (serving)", "value": r["keys_per_s"],
                "unit": "keys/s", "steps": r["steps"], "warmup": 0,
                "ms_per_step": 1e3 * r["wall_s"] / max(1, r["steps"]),
                "p50_s": r["p50_s"], "p99_s": r["p99_s"], "unfinished": r["unfinished"],
                "config": {"workload": "serving: Poisson 20/s for %.0f s, 5-9 hex digits (P=.30/.30/.20/.15/.05), "
-                                      "admission %s, %d ms steps, 2^%d keys per lane"
-                                      % (args.duration_s, args.policy, r["budget_ms"], r["lane_keys_log2"]),
+                                      "admission %s%s, %d ms steps, 2^%d keys per lane"
+                                      % (args.duration_s, args.policy, "" if args.no_preempt else " with preemption",
+                                         r["budget_ms"], r["lane_keys_log2"]),
                           "shape": r["shape"]}}
         out.update(common)
         out["result"] = r
@@ -321,6 +325,8 @@ def main():
     ap.add_argument("--policy", choices=("short-first", "short-first-oldest", "fifo"), default="short-first-oldest")
     ap.add_argument("--oldest-share-pct", type=int, default=20,
                     help="short-first-oldest: share of every step for the longest-waiting request")
+    ap.add_argument("--no-preempt", action="store_true",
+                    help="serving: admit only into free room (no suspension of lower-priority requests)")
     ap.add_argument("--serving-s", type=float, default=60.0,
                     help="plain workload: also run the serving loop for this long and report time-to-hit (0: skip)")
     ap.add_argument("--one-by-one-s", type=float, default=60.0,
@@ -399,13 +405,15 @@ def main():
     gpu_s = sum(kernel_ms) / 1e3
     value = keys / gpu_s
 
-    mad_peak, clock, cus = measure_mad_peak()
+    mad_peak, clock, cus, mad_wpc = measure_mad_peak()
     kpi = args.subwalks or b.batch  # keys per inversion
     work = mad_work_per_key(kpi, affine=bool(args.subwalks))
     achieved = value * work
     roofline = {"bound": "valu-imad", "achieved": achieved / 1e12, "peak": (mad_peak or 0) / 1e12,
                 "unit": "T mad/s", "frac": (achieved / mad_peak) if mad_peak else None,
-                "traffic": None, "mads_per_key": work}
+                "traffic": None, "mads_per_key": work,
+                "peak_source": "v_mad_u64_u32 issue-rate probe, 8 chains per lane, best of 16 and 32 waves/CU "
+                               "(%d), %.3f GHz measured" % (mad_wpc, clock / 1e9)}
     if pairs:
         roofline["mads_per_key_executed"] = pair_mads_per_key(kpi)
     # HBM: the algorithmic bytes per key are the batch arrays (a_i and Z_i,

@@ -322,6 +322,14 @@ typedef struct lxv_multi_opts {
                                shared by priority, and that search is admitted first;
                                bounds how long a low-priority search can starve.
                                0 (default): strict priority */
+    uint32_t preempt;       /* 1: a queued search that finds no free slot or pool range
+                               suspends resident searches of a lower priority (larger
+                               value) until it fits; a suspended search keeps the
+                               progress collected so far (a partly walked block is
+                               walked again) and is set up again at that block when it
+                               is readmitted, so its answer is unchanged. Searches
+                               walked by a pipelined step still in flight are not
+                               suspended. 0 (default): admission only into free room */
 } lxv_multi_opts;
 
 typedef struct lxv_search_spec {

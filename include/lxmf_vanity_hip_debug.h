@@ -44,12 +44,6 @@ int lxv_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint3
 int lxv_debug_address(const uint32_t ed[8], const uint32_t nh[3], const uint32_t* u, uint32_t* out, uint32_t n,
                       int on_gpu);
 
-/* Measures the issue rate of v_mad_u64_u32 (32 x 32 -> 64 multiply-add) on
- * `device` with a register-only probe kernel: lane-level mads per second, the
- * peak the walk's multiply count is priced against. Also returns the
- * device's clock and CU count. */
-int lxv_debug_mad_rate(int device, double* mads_per_s, double* clock_hz, int* cus);
-
 /* VALU issue-rate probes for lxv_debug_valu_rate: one instruction in 8
  * independent chains per lane, or the SHA-256 round mix. */
 #define LXV_VALU_ALIGNBIT 0     /* v_alignbit_b32 (rotates) */
@@ -59,7 +53,7 @@ int lxv_debug_mad_rate(int device, double* mads_per_s, double* clock_hz, int* cu
 #define LXV_VALU_XOR 4          /* v_xor_b32 */
 #define LXV_VALU_PERM 5         /* v_perm_b32 (byte swaps) */
 #define LXV_VALU_SHIFT 6        /* v_lshrrev_b32 */
-#define LXV_VALU_MAD64 7        /* v_mad_u64_u32 */
+#define LXV_VALU_MAD64 7        /* v_mad_u64_u32 (the peak the walk's multiply count is priced against) */
 #define LXV_VALU_SHA_MIX 8      /* 3 v_alignbit : 2 v_bitop3 : 2 v_add3, as a SHA-256 round */
 #define LXV_VALU_ADD_CO 9       /* v_add_co_u32 (carry out to VCC) */
 #define LXV_VALU_CARRY_CHAIN 10 /* v_addc_co_u32 chains through VCC, each reading the last's carry */

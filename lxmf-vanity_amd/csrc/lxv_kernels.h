@@ -10,6 +10,15 @@ namespace lxv {
 
 struct WalkParams;
 
+// The waves per SIMD a kernel is compiled for, from the resident waves per
+// CU asked for (4 SIMDs per CU): 16 and up -> 4, 8 -> 2, fewer -> 1.
+// (5 and 6 per SIMD, with 96 and 80 registers and more spilling, measured
+// +0.3% and -0.2% keys/s for the pair walk at B = 64 and less at B = 256:
+// not built.)
+inline int waves_per_simd(uint32_t waves_per_cu) {
+    return waves_per_cu >= 16 ? 4 : waves_per_cu >= 8 ? 2 : 1;
+}
+
 enum DebugFeOp {
     kFeMul = 0, kFeSq = 1, kFeAdd = 2, kFeSub = 3, kFeInvert = 4, kFeCanon = 5, kFeMulSmall = 6, kFeInvertChain = 7
 };
@@ -124,8 +133,6 @@ hipError_t launch_multi_init(int layout, int walk, uint32_t* pool, uint32_t pool
                              const PairAnchor* s_anchor, hipStream_t s);
 hipError_t launch_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n,
                            hipStream_t s);
-// lanes must be a multiple of 256; 8 mads per lane per iteration.
-hipError_t launch_mad_rate(uint32_t* out, uint32_t lanes, uint32_t seed, uint32_t iters, hipStream_t s);
 // VALU issue-rate probe `op` (LXV_VALU_* of lxmf_vanity_hip_debug.h) over
 // `blocks` blocks of 256 lanes: out[lane] (blocks x 256 words) and, per block,
 // 4 clock readings (shader clock at start and end, constant-rate clock at

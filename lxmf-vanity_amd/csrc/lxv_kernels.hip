@@ -96,28 +96,6 @@ __global__ void debug_addr_kernel(const ShaSearch c, const uint32_t* u, uint32_t
     for (int k = 0; k < 4; k++) out[i * 4 + k] = a[k];
 }
 
-// Issue-rate probe of v_mad_u64_u32 (the 32 x 32 -> 64 multiply-add the
-// field arithmetic is priced in): 8 independent accumulation chains per lane,
-// `iters` rounds. Reads and writes only out[lane].
-__global__ __launch_bounds__(256) void mad_rate_kernel(uint32_t* out, uint32_t seed, uint32_t iters) {
-    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t w0 = seed ^ lane, w1 = w0 * 3, w2 = w0 * 5, w3 = w0 * 7, w4 = w0 * 9, w5 = w0 * 11, w6 = w0 * 13,
-             w7 = w0 * 15;
-    const uint32_t a = seed + 1, b = seed * 7;
-    for (uint32_t it = 0; it < iters; it++) {
-        uint64_t c0, c1, c2, c3, c4, c5, c6, c7;
-        asm volatile(
-            "v_mad_u64_u32 %0, %8, %16, %17, %0\n v_mad_u64_u32 %1, %9, %16, %17, %1\n"
-            "v_mad_u64_u32 %2, %10, %16, %17, %2\n v_mad_u64_u32 %3, %11, %16, %17, %3\n"
-            "v_mad_u64_u32 %4, %12, %16, %17, %4\n v_mad_u64_u32 %5, %13, %16, %17, %5\n"
-            "v_mad_u64_u32 %6, %14, %16, %17, %6\n v_mad_u64_u32 %7, %15, %16, %17, %7"
-            : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5), "+v"(w6), "+v"(w7), "=s"(c0), "=s"(c1),
-              "=s"(c2), "=s"(c3), "=s"(c4), "=s"(c5), "=s"(c6), "=s"(c7)
-            : "v"(a), "v"(b));
-    }
-    out[lane] = (uint32_t)(w0 ^ w1 ^ w2 ^ w3 ^ w4 ^ w5 ^ w6 ^ (w7 >> 3));
-}
-
 // Issue-rate probes of single VALU instructions and of the SHA-256 mix, at
 // a chosen occupancy: 8 independent chains per lane (so a wave never waits on
 // its own results), each loop iteration kValuOpsPerIter[op] instructions,
@@ -428,11 +406,6 @@ hipError_t launch_debug_fe(int layout, int op, const uint32_t* a, const uint32_t
         hipLaunchKernelGGL(debug_fe_kernel<Field8>, grid, block, 0, s, op, a, b, out, n);
     else
         hipLaunchKernelGGL(debug_fe_kernel<Field10>, grid, block, 0, s, op, a, b, out, n);
-    return hipGetLastError();
-}
-
-hipError_t launch_mad_rate(uint32_t* out, uint32_t lanes, uint32_t seed, uint32_t iters, hipStream_t s) {
-    hipLaunchKernelGGL(mad_rate_kernel, dim3(lanes / 256), dim3(256), 0, s, out, seed, iters);
     return hipGetLastError();
 }
 

@@ -27,6 +27,7 @@ namespace {
 
 constexpr uint64_t kStride = 1ull << 40;   // key distance between virtual lanes
 constexpr uint32_t kNone = 0xffffffffu;
+constexpr size_t kMaxSuspendPerStep = 64;   // suspensions one admission may make
 
 thread_local std::string g_err = "no error";
 
@@ -54,6 +55,7 @@ struct Search {
     uint64_t hit_n = 0;
     uint64_t keys = 0;
     double submit_s = 0, admit_s = 0;
+    uint32_t suspensions = 0;
     bool active = false;
 };
 
@@ -133,6 +135,8 @@ struct lxv_multi {
     std::string err = "no error";
     int layout = LXV_LAYOUT_8X32, batch = 256, walk = LXV_WALK_PAIRS, waves = 4;
     double oldest_share = 0;  // of each step's work, for the longest-waiting resident search
+    bool preempt = false;     // a search that does not fit suspends resident ones of lower priority
+    uint64_t suspensions = 0;
     uint32_t rounds = 1, limbs = 8;
     uint32_t phys_lanes = 0, pool_lanes = 0, max_slots = 1024, max_vlanes = 1u << 18, lane_keys_log2 = 14;
     bool use_graph = true;
@@ -261,12 +265,13 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
     const lxv_tuned_shape& tuned = kTunedShapes[LXV_TUNED_BATCHED];
     m->batch = o.batch ? (int)o.batch : (int)tuned.batch;
     const uint32_t wpc = o.waves_per_cu ? o.waves_per_cu : tuned.waves_per_cu;
-    m->waves = wpc >= 16 ? 4 : (wpc >= 8 ? 2 : 1);
+    m->waves = waves_per_simd(wpc);
     if (o.oldest_share_pct > 90) {
         delete m;
         return mfail(nullptr, LXV_E_ARG, "lxv_multi_open: oldest_share_pct must be 0..90");
     }
     m->oldest_share = o.oldest_share_pct / 100.0;
+    m->preempt = o.preempt != 0;
     m->layout = (int)o.layout;
     m->walk = (int)o.walk;
     m->use_graph = !o.no_graph;
@@ -484,55 +489,134 @@ static void finish(lxv_multi* m, Search& s, int status, uint64_t g, bool found) 
 // not hold back smaller ones behind it. The admitted searches' slot tables
 // are copied from pinned host mirrors and their lanes set up by one ladder
 // launch, all enqueued ahead of the step's walk on the same stream.
+// Whether search r may be suspended to make room: resident, not admitted in
+// this call, not walked by the step still in flight (its progress there is
+// not collected yet), and not the longest-waiting resident search when that
+// one has a share.
+static bool suspendable(lxv_multi* m, const Search& r, const std::vector<uint64_t>& admitted_now,
+                        const Search* oldest_active) {
+    if (!r.active || &r == oldest_active) return false;
+    if (std::find(admitted_now.begin(), admitted_now.end(), r.id) != admitted_now.end()) return false;
+    if (m->pipelined && m->inflight)
+        for (const auto& mem : m->rec[1 - m->parity].members)
+            if (mem.first == r.id) return false;
+    return true;
+}
+
+// Suspends a resident search: its slot and pool lanes are freed and it goes
+// back to the queue with the progress collected so far. A partly walked
+// block is rolled back to its start (its keys are walked again after the
+// resumption, which sets every lane up at that block from the start
+// scalar), so its key count and answer are those of an uninterrupted walk.
+static void suspend(lxv_multi* m, Search& r) {
+    m->pool.give(r.state_off, r.vlanes);
+    m->slot_owner[r.slot] = 0;
+    m->free_slots.push_back(r.slot);
+    r.active = false;
+    r.slot = kNone;
+    r.next_block = r.done_block;
+    r.wave_cursor = 0;
+    r.done_cursor = 0;
+    r.suspensions++;
+    m->suspensions++;
+}
+
+// k / 8-based set-up scalar of a search resumed at key n0 of every lane:
+// base + n0 as 8 little-endian words.
+static void base_plus(const uint32_t base[8], uint64_t n0, uint32_t out[8]) {
+    uint64_t c = n0;
+    for (int i = 0; i < 8; i++) {
+        const uint64_t v = (uint64_t)base[i] + (c & 0xffffffffull);
+        out[i] = (uint32_t)v;
+        c = (c >> 32) + (v >> 32);
+    }
+}
+
+// Admits queued searches by (priority, submission order), each as soon as a
+// slot and a pool range of its size are free; one that does not fit yet does
+// not hold back smaller ones behind it. With `preempt`, a search that does
+// not fit suspends resident searches of a lower priority (a larger value),
+// lowest priority and latest submission first, until it fits. The admitted
+// searches' slot tables are copied from pinned host mirrors and their lanes
+// set up by one launch (at the block each has reached, for resumed ones),
+// all enqueued ahead of the step's walk on the same stream.
 static int admit(lxv_multi* m) {
-    if (m->queue.empty() || m->free_slots.empty()) return LXV_OK;
+    if (m->queue.empty()) return LXV_OK;
     if (m->admit_pending) {
         // the staging entries of the last admission are read by its copy
         MCHK(m, hipEventSynchronize(m->ev_admit));
         m->admit_pending = false;
     }
-    std::stable_sort(m->queue.begin(), m->queue.end(), [m](uint64_t a, uint64_t b) {
-        return m->searches.at(a).priority < m->searches.at(b).priority;
+    std::sort(m->queue.begin(), m->queue.end(), [m](uint64_t a, uint64_t b) {
+        const uint32_t pa = m->searches.at(a).priority, pb = m->searches.at(b).priority;
+        return pa != pb ? pa < pb : a < b;
     });
     if (m->oldest_share > 0) {
         // the longest-waiting queued search goes first, whatever its priority
         auto first = std::min_element(m->queue.begin(), m->queue.end());
         std::rotate(m->queue.begin(), first, first + 1);
     }
+    const Search* oldest_active = nullptr;
+    if (m->oldest_share > 0)
+        for (auto& kv : m->searches)
+            if (kv.second.active && (!oldest_active || kv.first < oldest_active->id)) oldest_active = &kv.second;
+    std::vector<uint64_t> admitted_now, suspended;
     uint32_t n = 0, max_vl = 0;
-    for (auto qi = m->queue.begin(); qi != m->queue.end() && !m->free_slots.empty();) {
+    for (auto qi = m->queue.begin(); qi != m->queue.end();) {
         Search& s = m->searches.at(*qi);
-        uint32_t off;
-        if (!m->pool.take(s.vlanes, &off)) {  // admission by device memory
+        uint32_t off = 0;
+        bool fits = !m->free_slots.empty() && m->pool.take(s.vlanes, &off);
+        while (!fits && m->preempt && suspended.size() < kMaxSuspendPerStep) {
+            Search* victim = nullptr;
+            for (auto& kv : m->searches) {
+                Search& r = kv.second;
+                if (r.priority <= s.priority || !suspendable(m, r, admitted_now, oldest_active)) continue;
+                if (!victim || r.priority > victim->priority || (r.priority == victim->priority && r.id > victim->id))
+                    victim = &r;
+            }
+            if (!victim) break;
+            suspend(m, *victim);
+            suspended.push_back(victim->id);
+            fits = !m->free_slots.empty() && m->pool.take(s.vlanes, &off);
+        }
+        if (!fits) {  // admission by device memory
             ++qi;
             continue;
         }
         qi = m->queue.erase(qi);
+        admitted_now.push_back(s.id);
         s.slot = m->free_slots.back();
         m->free_slots.pop_back();
         s.state_off = off;
         s.sc.state_off = off;
         s.sc.slot = s.slot;
         s.active = true;
-        s.admit_s = m->now();
+        if (s.suspensions == 0) s.admit_s = m->now();
         m->slot_owner[s.slot] = s.id;
         // a slot's host mirror is rewritten only here, after the step that
         // last read it has been waited for
         m->h_slots[s.slot] = s.sc;
         std::memset(&m->h_dyn[s.slot], 0, sizeof(SlotDyn));
-        m->h_dyn[s.slot].found_block = kNone;
+        m->h_dyn[s.slot].found_block = s.hit ? s.hit_block : kNone;
         MCHK(m, hipMemcpyAsync(m->d_slots + s.slot, m->h_slots + s.slot, sizeof(SlotConst), hipMemcpyHostToDevice,
                                m->stream));
         MCHK(m, hipMemcpyAsync(m->d_dyn + s.slot, m->h_dyn + s.slot, sizeof(SlotDyn), hipMemcpyHostToDevice,
                                m->stream));
         MultiInitEntry& e = m->h_init[n++];
-        std::memcpy(e.base, s.sc.base, sizeof e.base);
-        e.p0 = s.p0;
+        const uint64_t n0 = (uint64_t)s.next_block * m->rounds * (uint64_t)m->batch;
+        base_plus(s.sc.base, n0, e.base);
+        if (m->walk == LXV_WALK_PAIRS) {
+            if (n0 == 0)
+                e.p0 = s.p0;
+            else
+                pair_anchor<Field8>(e.base, e.p0);  // the resumed block's centre of virtual lane 0
+        }
         e.state_off = off;
         e.vlanes = s.vlanes;
         e.pad[0] = e.pad[1] = 0;
         max_vl = std::max(max_vl, s.vlanes);
     }
+    for (uint64_t id : suspended) m->queue.push_back(id);
     if (n) {
         MCHK(m, hipMemcpyAsync(m->d_init, m->h_init, (size_t)n * sizeof(MultiInitEntry), hipMemcpyHostToDevice,
                                m->stream));

@@ -184,7 +184,7 @@ extern "C" int lxv_open(const lxv_open_opts* opts, lxv_ctx** out) {
     }
     const lxv_tuned_shape& tuned = kTunedShapes[LXV_TUNED_SINGLE];
     uint32_t wpc = o.waves_per_cu ? o.waves_per_cu : tuned.waves_per_cu;
-    c->waves = wpc >= 16 ? 4 : (wpc >= 8 ? 2 : 1);
+    c->waves = (uint32_t)waves_per_simd(wpc);
     c->subwalks = o.subwalks;
     if (c->subwalks) {
         // K sub-walks per lane: a thread is a sub-walk, one key per iteration
@@ -714,34 +714,6 @@ extern "C" int lxv_debug_address(const uint32_t ed[8], const uint32_t nh[3], con
     return e == hipSuccess ? LXV_OK : LXV_E_HIP;
 }
 
-extern "C" int lxv_debug_mad_rate(int device, double* mads_per_s, double* clock_hz, int* cus) {
-    if (!mads_per_s) return LXV_E_ARG;
-    hipDeviceProp_t prop;
-    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) return LXV_E_HIP;
-    const uint32_t lanes = (uint32_t)prop.multiProcessorCount * 32u * 64u, iters = 8192;
-    uint32_t* out = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipError_t e = hipMalloc((void**)&out, (size_t)lanes * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipEventCreate(&e0);
-    if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e == hipSuccess) e = launch_mad_rate(out, lanes, 1, iters, nullptr);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    float ms = 0;
-    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
-    for (int r = 0; r < 5 && e == hipSuccess; r++) e = launch_mad_rate(out, lanes, 2 + r, iters, nullptr);
-    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    (void)hipFree(out);
-    if (e != hipSuccess || ms <= 0) return LXV_E_HIP;
-    *mads_per_s = 5.0 * lanes * (double)iters * 8.0 / (ms * 1e-3);
-    if (clock_hz) *clock_hz = prop.clockRate * 1e3;
-    if (cus) *cus = prop.multiProcessorCount;
-    return LXV_OK;
-}
-
 extern "C" int lxv_debug_valu_rate(int device, int op, uint32_t waves_per_cu, double* lane_ops_per_s,
                                    double* clock_hz, double* nominal_clock_hz, int* cus) {
     const int per_iter = valu_probe_ops_per_iter(op);
@@ -820,7 +792,7 @@ extern "C" int lxv_debug_copy(int device, uint64_t bytes, double* ms) {
 extern "C" int lxv_debug_kernel_attrs(int layout, uint32_t batch, uint32_t waves_per_cu, uint32_t walk,
                                       uint32_t* vgprs, uint32_t* sgprs, uint32_t* scratch_bytes,
                                       uint32_t* waves_per_simd) {
-    const int w = waves_per_cu >= 16 ? 4 : (waves_per_cu >= 8 ? 2 : 1);
+    const int w = lxv::waves_per_simd(waves_per_cu);
     const bool pairs = walk == LXV_WALK_PAIRS;
     if (pairs ? !walk_pair_supported(layout, (int)batch, w) : !walk_supported(layout, (int)batch, w))
         return LXV_E_ARG;

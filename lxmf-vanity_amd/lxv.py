@@ -67,7 +67,7 @@ class MultiOpts(ctypes.Structure):
                 ("batch", ctypes.c_uint32), ("layout", ctypes.c_uint32), ("no_graph", ctypes.c_uint32),
                 ("pool_bytes", ctypes.c_uint64), ("max_vlanes", ctypes.c_uint32), ("lane_keys_log2", ctypes.c_uint32),
                 ("walk", ctypes.c_uint32), ("pipelined", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32),
-                ("oldest_share_pct", ctypes.c_uint32)]
+                ("oldest_share_pct", ctypes.c_uint32), ("preempt", ctypes.c_uint32)]
 
 
 class TunedShape(ctypes.Structure):
@@ -421,10 +421,10 @@ class Multi:
 
     def __init__(self, max_searches=0, batch=0, layout=LAYOUT_8X32, no_graph=False, pool_bytes=0,
                  max_vlanes=0, lane_keys_log2=0, device=0, walk=WALK_PAIRS, pipelined=False, waves_per_cu=0,
-                 oldest_share_pct=0):
+                 oldest_share_pct=0, preempt=False):
         o = MultiOpts(ctypes.sizeof(MultiOpts), device, max_searches, batch, layout, 1 if no_graph else 0,
                       pool_bytes, max_vlanes, lane_keys_log2, walk, 1 if pipelined else 0, waves_per_cu,
-                      oldest_share_pct)
+                      oldest_share_pct, 1 if preempt else 0)
         h = ctypes.c_void_p()
         if lib().lxv_multi_open(ctypes.byref(o), ctypes.byref(h)) != LXV_OK:
             raise GpuUnavailable(lib().lxv_multi_last_error(None).decode())

@@ -181,14 +181,18 @@ POLICIES = ("short-first", "short-first-oldest", "fifo")
 
 
 def serving(rate=20.0, duration_s=60.0, policy="short-first-oldest", budget_ms=0, seed=4, lane_keys_log2=0,
-            pool_bytes=8 << 30, progress=print, oldest_share_pct=20, shape=None):
+            pool_bytes=8 << 30, progress=print, oldest_share_pct=20, shape=None, preempt=True):
     """The serving loop. Requests are admitted into lxv_multi in `policy`
     order: "short-first" (priority = digits: fewest expected keys, 16^digits,
     then arrival; strict, so under overload the longest requests starve),
     "short-first-oldest" (the same, but every step gives oldest_share_pct of
     its work to the longest-waiting resident request, which is also admitted
     first) or "fifo" (one priority: processor sharing in arrival order).
-    Each step walks about budget_ms of work over every resident request;
+    With preempt (the default), a request that finds the HBM pool or the
+    slots full suspends resident requests of a lower priority (the library
+    keeps their progress and sets their lanes up again when they return), so
+    long requests that wait for service cannot hold the memory that new short
+    ones need. Each step walks about budget_ms of work over every resident request;
     time-to-hit is from arrival to the answer being on the host. Measured on
     MI355X over 60 s (8x32, B = 256, 16 waves/CU): 15/25/35/50/100 ms steps
     gave p50 0.053/0.050/0.060/0.076/0.117 s at 87/92/94/92/95% of the plain
@@ -205,7 +209,7 @@ def serving(rate=20.0, duration_s=60.0, policy="short-first-oldest", budget_ms=0
     # over 60 s: 93.1% of the plain kernel's keys/s against 91.7%) but makes a
     # new request wait for the step already in flight (p50 0.112 s against
     # 0.076 s), and time-to-hit is what this workload is for.
-    m = open_multi(sh, pool_bytes=pool_bytes,
+    m = open_multi(sh, pool_bytes=pool_bytes, preempt=preempt,
                    oldest_share_pct=oldest_share_pct if policy == "short-first-oldest" else 0)
     try:
         by_id = {}
@@ -254,6 +258,7 @@ def serving(rate=20.0, duration_s=60.0, policy="short-first-oldest", budget_ms=0
         m.close()
     out = _latency_report(reqs, end)
     out.update({"policy": policy, "oldest_share_pct": oldest_share_pct if policy == "short-first-oldest" else 0,
+                "preempt": preempt,
                 "shape": sh, "rate_per_s": rate, "duration_s": duration_s, "wall_s": end,
                 "budget_ms": budget_ms, "lane_keys_log2": lane_keys_log2,
                 "keys": st["keys"], "busy_s": st["busy_s"], "steps": st["steps"],

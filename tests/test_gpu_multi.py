@@ -200,6 +200,48 @@ def test_oldest_share_bounds_the_wait_of_a_low_priority_search():
     check_first_hit(got[1], *slow)
 
 
+@pytest.mark.parametrize("walk,lane_bytes", [(lxv.WALK_PAIRS, 64), (lxv.WALK_PROJECTIVE, 128)],
+                         ids=["pairs", "projective"])
+def test_preempted_search_resumes_with_its_answer(walk, lane_bytes):
+    # Pool room for one 64-lane search. A 4-digit search at priority 9 (its
+    # first hit in block 9) walks a block per step; after 3 steps a 3-digit
+    # search at priority 0 arrives. With preemption it suspends the first,
+    # finishes first, and the first resumes at the block it reached and
+    # finishes with the oracle's first hit and a key count of an
+    # uninterrupted walk; without, the newcomer waits.
+    low, high = workload([7_100_007, 7_100_010], [4, 3])
+
+    def serve(preempt):
+        m = lxv.Multi(batch=B, waves_per_cu=16, lane_keys_log2=10, pool_bytes=64 * lane_bytes, walk=walk,
+                      preempt=preempt)
+        try:
+            lid = m.submit(*low, priority=9)
+            done = []
+            for _ in range(3):
+                done += m.step(5)
+            assert done == [] and m.stats()["resident"] == 1
+            hid = m.submit(*high, priority=0)
+            for _ in range(200):
+                done += m.step(5)
+                assert m.stats()["resident"] <= 1
+                if len(done) == 2:
+                    break
+            by = {r["id"]: r for r in done}
+            return [r["id"] for r in done], by[lid], by[hid]
+        finally:
+            m.close()
+
+    order, lo, hi = serve(False)
+    assert order == [order[0], order[1]] and lo["done_s"] < hi["done_s"]
+    order, lo2, hi2 = serve(True)
+    assert hi2["done_s"] < lo2["done_s"]
+    for r, spec in ((lo2, low), (hi2, high), (lo, low)):
+        assert r["status"] == 1
+        check_first_hit(r, *spec)
+    assert (lo2["g"], lo2["address"], lo2["keys"]) == (lo["g"], lo["address"], lo["keys"])
+    assert (lo2["g"] % STRIDE) // B == 9
+
+
 @pytest.mark.parametrize("walk,lane_bytes,pipelined", [(lxv.WALK_PAIRS, 64, False), (lxv.WALK_PROJECTIVE, 128, False),
                                                        (lxv.WALK_PAIRS, 64, True)],
                          ids=["pairs", "projective", "pairs-pipelined"])
