@@ -181,14 +181,17 @@ POLICIES = ("short-first", "short-first-oldest", "fifo")
 
 
 def serving(rate=20.0, duration_s=60.0, policy="short-first-oldest", budget_ms=0, seed=4, lane_keys_log2=0,
-            pool_bytes=8 << 30, progress=print, oldest_share_pct=20, shape=None, preempt=True):
+            pool_bytes=32 << 30, max_searches=4096, progress=print, oldest_share_pct=20, shape=None, preempt=True):
     """The serving loop. Requests are admitted into lxv_multi in `policy`
     order: "short-first" (priority = digits: fewest expected keys, 16^digits,
     then arrival; strict, so under overload the longest requests starve),
     "short-first-oldest" (the same, but every step gives oldest_share_pct of
     its work to the longest-waiting resident request, which is also admitted
     first) or "fifo" (one priority: processor sharing in arrival order).
-    With preempt (the default), a request that finds the HBM pool or the
+    The device-memory budget is 32 GiB of walk state (a 7-9 digit request
+    holds 16 MiB: 2^18 virtual lanes of 64 B) and 4096 slots, about two
+    thousand long requests, so under config 4's overload the pool fills only
+    late in a 600 s run. With preempt (the default), a request that finds the HBM pool or the
     slots full suspends resident requests of a lower priority (the library
     keeps their progress and sets their lanes up again when they return), so
     long requests that wait for service cannot hold the memory that new short
@@ -209,7 +212,7 @@ def serving(rate=20.0, duration_s=60.0, policy="short-first-oldest", budget_ms=0
     # over 60 s: 93.1% of the plain kernel's keys/s against 91.7%) but makes a
     # new request wait for the step already in flight (p50 0.112 s against
     # 0.076 s), and time-to-hit is what this workload is for.
-    m = open_multi(sh, pool_bytes=pool_bytes, preempt=preempt,
+    m = open_multi(sh, pool_bytes=pool_bytes, max_searches=max_searches, preempt=preempt,
                    oldest_share_pct=oldest_share_pct if policy == "short-first-oldest" else 0)
     try:
         by_id = {}
@@ -258,7 +261,7 @@ def serving(rate=20.0, duration_s=60.0, policy="short-first-oldest", budget_ms=0
         m.close()
     out = _latency_report(reqs, end)
     out.update({"policy": policy, "oldest_share_pct": oldest_share_pct if policy == "short-first-oldest" else 0,
-                "preempt": preempt,
+                "preempt": preempt, "pool_bytes": pool_bytes, "max_searches": max_searches,
                 "shape": sh, "rate_per_s": rate, "duration_s": duration_s, "wall_s": end,
                 "budget_ms": budget_ms, "lane_keys_log2": lane_keys_log2,
                 "keys": st["keys"], "busy_s": st["busy_s"], "steps": st["steps"],
