@@ -122,6 +122,15 @@ def cpu_baseline(seconds=12.0):
             "per_core": n / dt / cores, "cpu_model": cpu_model(), "cpus_visible": visible, "cores_reason": why}
 
 
+def split_iters(budget, per_iter, steps):
+    """The key budget in whole launch iterations (per_iter keys each), spread
+    over the timed steps so that exactly budget // per_iter iterations are
+    walked (at least one per step): each step gets the floor or the ceiling
+    of the mean. 2^36 keys at the default shape are 64 iterations of 2^30."""
+    total = max(steps, budget // per_iter)
+    return [total // steps + (1 if i < total % steps else 0) for i in range(steps)]
+
+
 def keys_text(k):
     return "2^%d" % (k.bit_length() - 1) if k & (k - 1) == 0 else "%.4g" % k
 
@@ -236,6 +245,23 @@ def plain_rate(batch=0, waves_per_cu=0, budget_log2=33):
         s.close()
 
 
+def batched_pass(args, plain_keys_per_s):
+    """The batched mix in the same run: 256 seeded searches (4-10 hex digits)
+    in the persistent kernel at the tuned batched shape for --batched-s
+    seconds; aggregate keys/s against this run's plain-kernel keys/s and
+    against the plain kernel at the batched kernel's own shape."""
+    import workloads
+    log = lambda msg: print(msg, file=sys.stderr, flush=True)  # noqa: E731
+    r = workloads.batched(wall_s=args.batched_s, progress=log)
+    sh = r["shape"]
+    same_shape = plain_rate(batch=sh["batch"], waves_per_cu=sh["waves_per_cu"])
+    return {"workload": "batched: 256 seeded searches, 4-10 hex digits (P=.10/.15/.20/.20/.15/.12/.08), "
+                        "%.0f s wall" % args.batched_s,
+            "keys_per_s": r["keys_per_s"], "kernel_keys_per_s": r["kernel_keys_per_s"], "found": r["found"],
+            "rechecked": r["rechecked"], "shape": sh, "keys_per_s_vs_plain": r["keys_per_s"] / plain_keys_per_s,
+            "plain_same_shape_keys_per_s": same_shape, "keys_per_s_vs_plain_same_shape": r["keys_per_s"] / same_shape}
+
+
 def serving_pass(args, plain_keys_per_s):
     """The metric's second half, in the same run: the serving loop over the
     Poisson request stream (20/s, 5-9 hex digits) for --serving-s seconds,
@@ -327,6 +353,8 @@ def main():
                     help="short-first-oldest: share of every step for the longest-waiting request")
     ap.add_argument("--no-preempt", action="store_true",
                     help="serving: admit only into free room (no suspension of lower-priority requests)")
+    ap.add_argument("--batched-s", type=float, default=20.0,
+                    help="plain workload: also run the 256-search batched mix for this long (0: skip)")
     ap.add_argument("--serving-s", type=float, default=60.0,
                     help="plain workload: also run the serving loop for this long and report time-to-hit (0: skip)")
     ap.add_argument("--one-by-one-s", type=float, default=60.0,
@@ -373,11 +401,8 @@ def main():
     b = search.backend
     per_iter = b.threads * b.batch
     budget = 1 << args.budget_log2
-    # the budget in whole launch iterations, spread over the timed steps so
-    # that exactly `total_iters` x per_iter keys are walked (2^36 at the
-    # default shape: 64 iterations of 2^30 keys); steps get floor or ceil
-    total_iters = max(args.steps, budget // per_iter)
-    step_iters = [total_iters // args.steps + (1 if i < total_iters % args.steps else 0) for i in range(args.steps)]
+    step_iters = split_iters(budget, per_iter, args.steps)
+    total_iters = sum(step_iters)
     iters = min(step_iters)   # warm-up launches: the common size, so profiles average close to the timed ones
     mean_iters = total_iters / args.steps
     a = lxv.launch_args_for(ed_pub, prefix, iters=iters, max_hits=lxv.MAX_HITS)
@@ -475,6 +500,8 @@ def main():
         L.lxv_debug_copy(0, args.copy_probe_mib << 20, ctypes.byref(ms))
         out["copy_probe"] = {"bytes": args.copy_probe_mib << 20, "ms": ms.value}
     search.close()
+    if args.batched_s > 0:
+        out["batched"] = batched_pass(args, value)
     if args.serving_s > 0:
         out["time_to_hit"] = serving_pass(args, value)
     if not args.no_cpu:

@@ -179,7 +179,9 @@ int lxv_poll(lxv_ctx* ctx, int wait, lxv_hit* hits, uint32_t cap, uint32_t* n_hi
  * one GPU share that scratch: calls that overlap on one GPU (two streams,
  * or a launch of this kind still running) race. No lxv_poll follows: the
  * caller waits on its stream and reads its outputs itself; the context's
- * lxv_last_stats are not updated. */
+ * lxv_last_stats are not updated. lxv_stop ends it as it ends lxv_launch;
+ * the stop flag is cleared when this function is called (so not again when
+ * a captured graph of it is replayed). */
 int lxv_launch_noalloc(lxv_ctx* ctx, const lxv_launch_args* args, lxv_hit* d_hits, uint32_t* d_hit_count,
                        void* stream);
 

@@ -12,6 +12,12 @@
 //                  (lxv_subwalk.h), with its two initialisers
 //   debug kernels  field operations and the address hash on given inputs, for
 //                  parity tests against the CPU
+//   valu_rate      VALU issue-rate probes (single instruction classes, the
+//                  SHA-256 mix, the pair walk's mix) with the in-kernel clock
+//   copy           a plain copy of known size, to calibrate HBM byte counters
+//
+// The batched search's kernels are in lxv_multi_kernels.hip, the pair walk's
+// in lxv_pair_kernels.hip.
 #include <cstring>
 
 #include "fe10.h"

@@ -51,3 +51,25 @@ def test_pmc_summary_scales_the_counters_by_the_copy_probe(tmp_path):
     assert abs(out["hbm_read_bytes_per_key"] - 20.0) < 1e-9 and abs(out["hbm_write_bytes_per_key"] - 16.0) < 1e-9
     assert abs(out["valu_busy_pct"] - 100.0 * 1e9 * 4 / (1024 * 1e6)) < 1e-9
     assert json.load(open(tmp_path / "walk_pmc.json")) == out
+
+
+def test_the_key_budget_is_walked_exactly():
+    sys.path.insert(0, ROOT)
+    import bench
+    per_iter = (1 << 24) * 64          # the default grid's lanes x B = 64
+    for steps in (1, 8, 20, 64):
+        it = bench.split_iters(1 << 36, per_iter, steps)
+        assert len(it) == steps and sum(it) * per_iter == 1 << 36 and max(it) - min(it) <= 1
+    assert bench.split_iters(1 << 36, per_iter, 20)[:5] == [4, 4, 4, 4, 3]
+    assert bench.keys_text(1 << 36) == "2^36"
+
+
+def test_host_cores_follow_the_lease(monkeypatch):
+    sys.path.insert(0, ROOT)
+    import bench
+    visible = len(os.sched_getaffinity(0))
+    monkeypatch.setenv("OMP_NUM_THREADS", "1")
+    cores, vis, why = bench.host_cores()
+    assert vis == visible and cores == (1 if visible > 1 else visible)
+    monkeypatch.delenv("OMP_NUM_THREADS")
+    assert bench.host_cores()[0] == visible

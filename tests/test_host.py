@@ -78,6 +78,16 @@ def test_tuned_shapes_are_the_tables_picks():
             assert got[k] == picks[w][j], (w, k)
 
 
+def test_workloads_launch_at_the_tuned_shapes():
+    # the serving loop and the batched workload open lxv_multi at the shape
+    # the library was built with, unless the caller overrides a field
+    import workloads
+    for w in ("batched", "serving"):
+        assert workloads.shape_for(w) == lxv.tuned_shape(w)
+    sh = workloads.shape_for("serving", step_ms=50, batch=0)
+    assert sh["step_ms"] == 50 and sh["batch"] == lxv.tuned_shape("serving")["batch"]
+
+
 def test_tuned_header_follows_the_table(tmp_path):
     # a changed table gives a changed header (the build regenerates it)
     import json
