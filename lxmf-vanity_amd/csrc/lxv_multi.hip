@@ -122,7 +122,6 @@ struct StepRec {
 struct lxv_multi {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipStream_t rstream = nullptr;  // read-back of a finished step while the next one runs
     // Steps alternate between two parities, each with its own events, hit
     // buffer, counters, unit staging and graph, so that with `pipelined` the
     // next step is planned and enqueued before the last one is collected.
@@ -161,8 +160,13 @@ struct lxv_multi {
     Unit* h_units[2] = {nullptr, nullptr};
     uint32_t* h_nunits[2] = {nullptr, nullptr};
     uint32_t units_cap = 0;
-    unsigned long long* h_best = nullptr;  // pinned, max_slots
-    uint32_t* h_counters = nullptr;  // pinned, 16 words
+    // What the host reads of a finished step, written by the step's own
+    // publish kernel into host-mapped memory (one block per parity): words
+    // [0, 4) the counters, then max_slots best keys. Reading it needs no copy,
+    // so the read-back of one step never waits behind the next one's kernel,
+    // which holds every CU.
+    uint32_t* h_pub[2] = {nullptr, nullptr};
+    uint32_t* d_pub[2] = {nullptr, nullptr};
     // graphs, one per parity
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -184,6 +188,10 @@ struct lxv_multi {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
 };
+
+// Words of one parity's host-mapped read-back block: 4 counters, then a
+// 64-bit best key per slot.
+static size_t pub_words(const lxv_multi* m) { return 4 + 2 * (size_t)m->max_slots; }
 
 static int mfail(lxv_multi* m, int code, const std::string& msg) {
     if (m)
@@ -227,10 +235,9 @@ static void multi_free(lxv_multi* m) {
         if (m->ev0[p]) (void)hipEventDestroy(m->ev0[p]);
         if (m->ev1[p]) (void)hipEventDestroy(m->ev1[p]);
     }
-    if (m->h_best) (void)hipHostFree(m->h_best);
-    if (m->h_counters) (void)hipHostFree(m->h_counters);
+    for (int p = 0; p < 2; p++)
+        if (m->h_pub[p]) (void)hipHostFree(m->h_pub[p]);
     if (m->stream) (void)hipStreamDestroy(m->stream);
-    if (m->rstream) (void)hipStreamDestroy(m->rstream);
     delete m;
 }
 
@@ -349,11 +356,12 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
     if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_dyn, (size_t)m->max_slots * sizeof(SlotDyn), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_init, (size_t)m->max_slots * sizeof(MultiInitEntry), hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev_admit, hipEventDisableTiming);
-    if (e == hipSuccess)
-        e = hipHostMalloc((void**)&m->h_best, (size_t)m->max_slots * sizeof(unsigned long long), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_counters, 64, hipHostMallocDefault);
+    for (int p = 0; p < 2 && e == hipSuccess; p++) {
+        e = hipHostMalloc((void**)&m->h_pub[p], pub_words(m) * sizeof(uint32_t),
+                          hipHostMallocMapped | hipHostMallocCoherent);
+        if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&m->d_pub[p], m->h_pub[p], 0);
+    }
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->rstream, hipStreamNonBlocking);
     for (int p = 0; p < 2 && e == hipSuccess; p++) {
         e = hipEventCreate(&m->ev0[p]);
         if (e == hipSuccess) e = hipEventCreate(&m->ev1[p]);
@@ -635,6 +643,27 @@ __global__ void fill_ones_kernel(uint32_t* words, uint32_t n) {
     if (i < n) words[i] = 0xffffffffu;
 }
 
+// Copies the step's counters (4 words) and its best key per slot into the
+// host-mapped block `out` with plain stores, after the walk kernel in stream
+// order.
+__global__ void publish_kernel(const uint32_t* counters, const unsigned long long* best, uint32_t slots,
+                               uint32_t* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 4) out[i] = counters[i];
+    if (i < slots) {
+        const unsigned long long k = best[i];
+        out[4 + 2 * i] = (uint32_t)k;
+        out[5 + 2 * i] = (uint32_t)(k >> 32);
+    }
+}
+
+static hipError_t publish(lxv_multi* m, int p) {
+    const uint32_t n = std::max<uint32_t>(m->max_slots, 4);
+    hipLaunchKernelGGL(publish_kernel, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->d_counters + 16 * p,
+                       m->d_best + (size_t)m->max_slots * p, m->max_slots, m->d_pub[p]);
+    return hipGetLastError();
+}
+
 static hipError_t clear_best(lxv_multi* m, int p) {
     const uint32_t n = 2 * m->max_slots;
     hipLaunchKernelGGL(fill_ones_kernel, dim3((n + 255) / 256), dim3(256), 0, m->stream,
@@ -653,6 +682,7 @@ static int launch_step(lxv_multi* m, int p) {
             hipError_t e = launch_clear(counters, 16, m->stream);
             if (e == hipSuccess) e = clear_best(m, p);
             if (e == hipSuccess) e = launch_multi(m->layout, m->batch, m->walk, m->waves, m->params[p], m->stream);
+            if (e == hipSuccess) e = publish(m, p);
             hipError_t e2 = hipStreamEndCapture(m->stream, &m->graph[p]);
             if (e != hipSuccess) return mfail(m, LXV_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
             MCHK(m, e2);
@@ -666,6 +696,7 @@ static int launch_step(lxv_multi* m, int p) {
         MCHK(m, clear_best(m, p));
         MCHK(m, hipEventRecord(m->ev0[p], m->stream));
         MCHK(m, launch_multi(m->layout, m->batch, m->walk, m->waves, m->params[p], m->stream));
+        MCHK(m, publish(m, p));
         MCHK(m, hipEventRecord(m->ev1[p], m->stream));
     }
     return LXV_OK;
@@ -810,13 +841,10 @@ static int collect_step(lxv_multi* m, int p) {
     MCHK(m, hipEventSynchronize(m->ev1[p]));
     float ms = 0;
     MCHK(m, hipEventElapsedTime(&ms, m->ev0[p], m->ev1[p]));
-    // on the read-back stream, so a step enqueued since is not waited for
-    MCHK(m, hipMemcpyAsync(m->h_counters, m->d_counters + 16 * p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                           m->rstream));
-    MCHK(m, hipMemcpyAsync(m->h_best, m->d_best + (size_t)m->max_slots * p, m->max_slots * sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, m->rstream));
-    MCHK(m, hipStreamSynchronize(m->rstream));
-    const uint64_t kkeys = (uint64_t)m->h_counters[2] | (uint64_t)m->h_counters[3] << 32;
+    // the step's publish kernel has written its counters and best keys to
+    // host-mapped memory before ev1[p]
+    const volatile uint32_t* pub = m->h_pub[p];
+    const uint64_t kkeys = (uint64_t)pub[2] | (uint64_t)pub[3] << 32;
     m->steps++;
     m->keys += kkeys;
     m->busy_s += ms / 1e3;
@@ -832,7 +860,8 @@ static int collect_step(lxv_multi* m, int p) {
         walked.push_back(s);
         // keep the lowest hit in walk order: block, then n, then v (the key
         // orders by (n, v), and n determines the block)
-        const unsigned long long key = m->h_best[mem.second];
+        const unsigned long long key = (unsigned long long)pub[4 + 2 * mem.second] |
+                                       (unsigned long long)pub[5 + 2 * mem.second] << 32;
         if (key == kNoHit) continue;
         const uint64_t n = key >> kHitVlaneBits;
         const uint32_t v = (uint32_t)(key & ((1u << kHitVlaneBits) - 1));
