@@ -190,10 +190,11 @@ def serving(rate=20.0, duration_s=60.0, policy="short-first-oldest", budget_ms=0
     first) or "fifo" (one priority: processor sharing in arrival order).
     The device-memory budget is 32 GiB of walk state (a 7-9 digit request
     holds 16 MiB: 2^18 virtual lanes of 64 B) and 4096 slots, about two
-    thousand long requests, so under config 4's overload the pool fills only
-    late in a 600 s run. With preempt (the default), a request that finds the HBM pool or the
-    slots full suspends resident requests of a lower priority (the library
-    keeps their progress and sets their lanes up again when they return), so
+    thousand long requests, so under the 9x overload of 20 requests/s the
+    pool fills only late in a 600 s run. With preempt (the default), a
+    request that finds the HBM pool or the slots full suspends resident
+    requests of a lower priority (the library keeps their progress and sets
+    their lanes up again when they return), so
     long requests that wait for service cannot hold the memory that new short
     ones need. Each step walks about budget_ms of work over every resident request;
     time-to-hit is from arrival to the answer being on the host. Measured on

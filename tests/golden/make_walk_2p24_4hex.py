@@ -1,4 +1,4 @@
-"""Writes tests/golden/config0.json: the small CPU workload's expected output.
+"""Writes tests/golden/walk_2p24_4hex.json: the small CPU workload's expected output.
 
 Workload: seed 0 of the seeded generator gives the start scalar k, the
 Ed25519 seed and a 4-hex-digit prefix; walk the 2^24 consecutive keys
@@ -10,7 +10,7 @@ oracle/Makefile). Before it is written it is cross-checked against the
 independent Python big-integer walk of textbook.py on the first 2^16 keys:
 the oracle's hits and XOR over that prefix of the range must equal Python's.
 
-Run from the repository root:  python tests/golden/make_config0.py
+Run from the repository root:  python tests/golden/make_walk_2p24_4hex.py
 """
 
 import json
@@ -59,9 +59,9 @@ def main():
     out = {"seed": SEED, "start": start.hex(), "ed_seed": ed_seed.hex(), "ed_pub": ed_pub.hex(),
            "prefix": prefix, "keys": KEYS, "xor": xor.hex(), "hits": [[g, a.hex()] for g, a in hits],
            "checked_prefix_keys": CHECK_KEYS, "checked_prefix_xor": py_xor.hex()}
-    with open(os.path.join(HERE, "config0.json"), "w") as f:
+    with open(os.path.join(HERE, "walk_2p24_4hex.json"), "w") as f:
         json.dump(out, f, indent=0)
-    print("config0.json: %d hits for %s over 2^24 keys" % (len(hits), prefix))
+    print("walk_2p24_4hex.json: %d hits for %s over 2^24 keys" % (len(hits), prefix))
 
 
 if __name__ == "__main__":

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 STRIDE = lxv.STRIDE_MULTI
 B = 256
-CONFIG3_LENGTHS = ((4, .10), (5, .15), (6, .20), (7, .20), (8, .15), (9, .12), (10, .08))
+MIX_LENGTHS_4_TO_10 = ((4, .10), (5, .15), (6, .20), (7, .20), (8, .15), (9, .12), (10, .08))
 
 
 def workload(seeds, digits):
@@ -91,7 +91,7 @@ def test_answers_are_first_hits_in_walk_order(kernel):
     assert st["done"] == len(searches) and st["keys"] > 0
 
 
-def test_config3_shaped_mix_of_256_searches():
+def test_mix_of_256_searches_of_4_to_10_digits():
     # The batched workload's shape: 256 seeded searches with prefix lengths
     # drawn from its 4-10 digit table, each with its own start and Ed25519
     # key, in one persistent kernel. Searches give up after about 2^24 keys,
@@ -101,7 +101,7 @@ def test_config3_shaped_mix_of_256_searches():
     digits = []
     for _ in range(256):
         x, acc = rng.random(), 0.0
-        for d, pr in CONFIG3_LENGTHS:
+        for d, pr in MIX_LENGTHS_4_TO_10:
             acc += pr
             if x < acc:
                 break

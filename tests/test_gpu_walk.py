@@ -343,12 +343,12 @@ def test_stop_ends_a_long_launch_and_the_walk_resumes(small_backend):
                                   dict(threads=65536, batch=256, layout=lxv.LAYOUT_10X26, walk=lxv.WALK_PROJECTIVE),
                                   dict(threads=1024, subwalks=64)],
                          ids=["8x32", "10x26", "8x32-queue", "10x26-queue", "8x32-proj", "10x26-proj", "8x32-sw64"])
-def test_config0_workload_matches_golden(form):
+def test_2p24_key_walk_matches_golden(form):
     # The small workload at full size: 2^24 consecutive keys of workload
     # seed 0 split evenly over the threads (thread t walks g in [m t, m t + m)
     # with m = 2^24 / threads), every hit of the 4-hex prefix and the XOR of
     # all 2^24 addresses against the committed fixture.
-    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config0.json")) as f:
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "walk_2p24_4hex.json")) as f:
         fx = json.load(f)
     start, ed_pub = bytes.fromhex(fx["start"]), bytes.fromhex(fx["ed_pub"])
     b = lxv.Backend(**form)
@@ -365,7 +365,7 @@ def test_config0_workload_matches_golden(form):
         b.close()
 
 
-def test_config1_subrange_checksum_matches_oracle():
+def test_headline_shape_2p28_subrange_matches_oracle():
     # The headline workload's shape (workload seed 1, 7-hex prefix, B = 64,
     # 2^22 lanes at 2^40 keys apart on the persistent grid of every resident
     # wave): a 2^28-key sub-range, lanes 0..L-1 x n < 2^28 / L, walked on the

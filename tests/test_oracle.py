@@ -250,11 +250,11 @@ def test_scan_matches_golden(seed):
         assert [(g, a.hex()) for g, a in hits] == [tuple(h) for h in fx["hits"]]
 
 
-def test_config0_workload_matches_golden():
+def test_2p24_key_walk_matches_golden():
     # 2^24 consecutive keys of workload seed 0, 4-hex prefix: every hit and
-    # the XOR of all addresses (tests/golden/config0.json, cross-checked
+    # the XOR of all addresses (tests/golden/walk_2p24_4hex.json, cross-checked
     # against the Python walk on its first 2^16 keys when it was written)
-    with open(os.path.join(GOLDEN, "config0.json")) as f:
+    with open(os.path.join(GOLDEN, "walk_2p24_4hex.json")) as f:
         fx = json.load(f)
     start = bytes.fromhex(fx["start"])
     assert start == tb.seeded_start(0)
