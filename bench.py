@@ -273,8 +273,8 @@ def serving_pass(args, plain_keys_per_s):
     r = workloads.serving(duration_s=args.serving_s, policy=args.policy, oldest_share_pct=args.oldest_share_pct,
                           preempt=not args.no_preempt, progress=log)
     keep = ("requests", "finished", "unfinished", "p50_s", "p99_s", "p50_s_with_unfinished", "p99_s_with_unfinished",
-            "policy", "oldest_share_pct", "preempt", "pool_bytes", "max_searches", "shape", "keys_per_s", "busy_s",
-            "steps", "rechecked", "per_length")
+            "policy", "oldest_share_pct", "preempt", "suspensions", "pool_bytes", "max_searches", "shape", "keys_per_s",
+            "busy_s", "steps", "rechecked", "per_length")
     out = {k: r[k] for k in keep}
     out["workload"] = ("serving: Poisson 20/s for %.0f s, 5-9 hex digits (P=.30/.30/.20/.15/.05), a fresh seeded "
                        "identity per request, admission %s" % (args.serving_s, args.policy))

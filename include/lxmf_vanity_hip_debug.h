@@ -113,6 +113,10 @@ int lxv_debug_pair_centre(int layout, uint32_t batch, const uint8_t start[32], u
  * (batches x 256 x 32 bytes). CPU only. */
 int lxv_debug_cpu_walk(const uint8_t start[32], uint32_t batches, uint8_t* out);
 
+/* Searches suspended so far by lxv_multi's preemption (lxv_multi_opts.preempt). */
+typedef struct lxv_multi lxv_multi;
+int lxv_debug_multi_suspensions(lxv_multi* m, uint64_t* n);
+
 #ifdef __cplusplus
 }
 #endif

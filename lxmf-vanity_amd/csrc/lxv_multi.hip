@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/lxmf_vanity_hip.h"
+#include "../../include/lxmf_vanity_hip_debug.h"
 #include "fe10.h"
 #include "fe8.h"
 #include "host_crypto.h"
@@ -529,8 +530,9 @@ static void suspend(lxv_multi* m, Search& r) {
     m->suspensions++;
 }
 
-// k / 8-based set-up scalar of a search resumed at key n0 of every lane:
-// base + n0 as 8 little-endian words.
+// The set-up scalar of a search resumed at key index n0 of every virtual
+// lane: base + n0 as 8 little-endian words (base = start / 8 + B/2 - 1 for
+// the pair walk, start / 8 - 1 for the projective walk).
 static void base_plus(const uint32_t base[8], uint64_t n0, uint32_t out[8]) {
     uint64_t c = n0;
     for (int i = 0; i < 8; i++) {
@@ -950,5 +952,11 @@ extern "C" int lxv_multi_stats(lxv_multi* m, lxv_multi_stat* out) {
 }
 
 extern "C" void lxv_multi_close(lxv_multi* m) { multi_free(m); }
+
+extern "C" int lxv_debug_multi_suspensions(lxv_multi* m, uint64_t* n) {
+    if (!m || !n) return LXV_E_ARG;
+    *n = m->suspensions;
+    return LXV_OK;
+}
 
 extern "C" const char* lxv_multi_last_error(lxv_multi* m) { return m ? m->err.c_str() : g_err.c_str(); }

@@ -460,6 +460,13 @@ class Multi:
         self._check(lib().lxv_multi_stats(self.h, ctypes.byref(s)), "stats")
         return {k: getattr(s, k) for k, _ in MultiStat._fields_}
 
+    def suspensions(self):
+        """Searches suspended so far by preemption."""
+        n = ctypes.c_uint64(0)
+        lib().lxv_debug_multi_suspensions.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
+        self._check(lib().lxv_debug_multi_suspensions(self.h, ctypes.byref(n)), "suspensions")
+        return n.value
+
     def run_all(self, budget_ms=200, timeout_s=600.0):
         """Steps until every submitted search is finished; returns results."""
         res, t0 = [], time.time()

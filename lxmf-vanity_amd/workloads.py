@@ -258,11 +258,13 @@ def serving(rate=20.0, duration_s=60.0, policy="short-first-oldest", budget_ms=0
                      % (now, len(reqs), fin, st["queued"], st["resident"]))
         end = time.perf_counter() - t0
         st = m.stats()
+        suspensions = m.suspensions()
     finally:
         m.close()
     out = _latency_report(reqs, end)
     out.update({"policy": policy, "oldest_share_pct": oldest_share_pct if policy == "short-first-oldest" else 0,
-                "preempt": preempt, "pool_bytes": pool_bytes, "max_searches": max_searches,
+                "preempt": preempt, "suspensions": suspensions, "pool_bytes": pool_bytes,
+                "max_searches": max_searches,
                 "shape": sh, "rate_per_s": rate, "duration_s": duration_s, "wall_s": end,
                 "budget_ms": budget_ms, "lane_keys_log2": lane_keys_log2,
                 "keys": st["keys"], "busy_s": st["busy_s"], "steps": st["steps"],

@@ -227,14 +227,14 @@ def test_preempted_search_resumes_with_its_answer(walk, lane_bytes):
                 if len(done) == 2:
                     break
             by = {r["id"]: r for r in done}
-            return [r["id"] for r in done], by[lid], by[hid]
+            return m.suspensions(), by[lid], by[hid]
         finally:
             m.close()
 
-    order, lo, hi = serve(False)
-    assert order == [order[0], order[1]] and lo["done_s"] < hi["done_s"]
-    order, lo2, hi2 = serve(True)
-    assert hi2["done_s"] < lo2["done_s"]
+    suspended, lo, hi = serve(False)
+    assert suspended == 0 and lo["done_s"] < hi["done_s"]
+    suspended, lo2, hi2 = serve(True)
+    assert suspended == 1 and hi2["done_s"] < lo2["done_s"]
     for r, spec in ((lo2, low), (hi2, high), (lo, low)):
         assert r["status"] == 1
         check_first_hit(r, *spec)
