@@ -86,8 +86,8 @@ def cpu_model():
 def host_cores():
     """(threads to use, CPUs visible, why). One thread per core of the lease,
     as the reference runs one search thread per core (available_parallelism,
-    src/main.rs:474-477). A GPU box leases a share of its host's CPUs per GPU
-    and says so in OMP_NUM_THREADS; sched_getaffinity may show the whole
+    src/main.rs:474-477). A shared GPU host may lease a share of its CPUs per
+    GPU and say so in OMP_NUM_THREADS; sched_getaffinity may show the whole
     host there, so the lease's figure wins when it is smaller."""
     visible = os.cpu_count() or 1
     try:

@@ -6,8 +6,8 @@
  * reference drives its kernels through the Rust trait `Backend`
  * (src/gpu/mod.rs:40-52), implemented for CUDA in src/gpu/cuda.rs:71-133,
  * with the kernel ABI of src/gpu/kernel.cu:21-36. The entry points below map
- * onto it one to one (each cites the interface it replaces); a Rust binding
- * for them is given in INTEGRATION.md.
+ * onto it one to one (each cites the interface it replaces); the Rust
+ * binding for them is lxmf-vanity_amd/rust/hip.rs.
  *
  * Conventions (as the reference's backend, made explicit for a C ABI):
  *  - Every call returns LXV_OK (0) or a negative LXV_E_* status; nothing
