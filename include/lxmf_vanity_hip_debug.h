@@ -24,16 +24,18 @@ extern "C" {
 #define LXV_FE_CANON 5
 #define LXV_FE_MUL121665 6
 #define LXV_FE_INVERT_CHAIN 7  /* z^(p-2) by the reference's addition chain (LXV_FE_INVERT: divsteps) */
-/* Or-ed into an op (LXV_LAYOUT_8X32 only): inputs are raw limbs (any value
- * below 2^256, bit 255 kept), and the output is the limbs the operation
- * produced (congruent to the result mod p, not canonical), except for
- * LXV_FE_CANON, which still returns the canonical encoding. Sums and
+/* Or-ed into an op: inputs are the layout's raw limbs (8 words per element
+ * for LXV_LAYOUT_8X32, any value below 2^256 with bit 255 kept; 10 words for
+ * LXV_LAYOUT_10X26, within each operation's input bounds), and the output is
+ * the limbs the operation produced (congruent to the result mod p, not
+ * canonical), except for LXV_FE_CANON, which still returns the canonical
+ * encoding (in the first 8 words of the element, the rest 0). Sums and
  * differences are returned as computed, not multiplied by one. */
 #define LXV_FE_RAW_LIMBS 0x100
 
 /* out[i] = canonical encoding of op(a[i], b[i]) in limb layout `layout`
  * (LXV_LAYOUT_*), for n elements of 8 little-endian words each (bit 255 of
- * the inputs dropped; see LXV_FE_RAW_LIMBS for the 8 x 32 layout's own
+ * the inputs dropped; see LXV_FE_RAW_LIMBS for each layout's own
  * representation). on_gpu selects the device kernel (device 0) or the
  * host build of the same code. Returns LXV_OK or a negative status. */
 int lxv_debug_fe(int layout, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n, int on_gpu);
@@ -58,6 +60,9 @@ int lxv_debug_address(const uint32_t ed[8], const uint32_t nh[3], const uint32_t
 #define LXV_VALU_ADD_CO 9       /* v_add_co_u32 (carry out to VCC) */
 #define LXV_VALU_CARRY_CHAIN 10 /* v_addc_co_u32 chains through VCC, each reading the last's carry */
 #define LXV_VALU_WALK_MIX 11    /* the instruction classes of the pair walk's per-key loop, in its proportions */
+#define LXV_VALU_MUL_LO 12      /* v_mul_lo_u32 */
+#define LXV_VALU_LSHL_ADD 13    /* v_lshl_add_u32 (shift and add, as in x * 19 = (x << 4) + 3x) */
+#define LXV_VALU_LSHL_OR 14     /* v_lshl_or_b32 (packing limbs into words) */
 
 /* Measures the issue rate of probe `op` on `device` with one resident round
  * of waves_per_cu waves per CU (a multiple of 4, at most 32): lane-level

@@ -22,28 +22,33 @@ inline int waves_per_simd(uint32_t waves_per_cu) {
 enum DebugFeOp {
     kFeMul = 0, kFeSq = 1, kFeAdd = 2, kFeSub = 3, kFeInvert = 4, kFeCanon = 5, kFeMulSmall = 6, kFeInvertChain = 7
 };
-// Or-ed into an op code (8 x 32 layout only): the inputs are taken as raw
-// limbs, every value below 2^256 including those at or above 2^255, and the
-// result is returned as the limbs the operation produced (weakly reduced,
-// congruent to the result) instead of its canonical encoding; kFeCanon still
-// returns the canonical encoding of its raw input.
+// Or-ed into an op code: the inputs are taken as the layout's raw limbs
+// (F::kLimbs words per element) and the result is returned as the limbs the
+// operation produced (congruent to the result, not reduced further) instead
+// of its canonical encoding; kFeCanon still returns the canonical encoding of
+// its raw input. For 8 x 32 the limbs are any value below 2^256, bit 255
+// included; for 10 x 25.5 they must keep to each operation's input bounds
+// (fe10.h).
 constexpr int kFeRawLimbs = 0x100;
+
+// Words per element in the debug buffers: 8 for encodings, F::kLimbs for raw
+// limbs.
+template <class F>
+LXV_HD constexpr int debug_fe_stride(int op) {
+    return (op & kFeRawLimbs) ? F::kLimbs : 8;
+}
 
 // out = canonical words of op(a, b); a and b are decoded as 8 little-endian
 // words (bit 255 dropped). Sums and differences are multiplied by one before
 // encoding, as the walk only encodes multiplication results. With
-// kFeRawLimbs (F = Field8): raw limbs in and out, see above.
+// kFeRawLimbs: raw limbs in and out, see above (a, b and out then hold
+// debug_fe_stride words; a canonical result fills the first 8, the rest 0).
 template <class F>
-LXV_HD void debug_fe_op(int op, const uint32_t a[8], const uint32_t b[8], uint32_t out[8]) {
+LXV_HD void debug_fe_op(int op, const uint32_t* a, const uint32_t* b, uint32_t* out) {
     typedef typename F::fe fe;
-    const bool raw = (op & kFeRawLimbs) != 0 && F::kLimbs == 8;
+    const bool raw = (op & kFeRawLimbs) != 0;
     op &= 0xff;
-    uint32_t la[F::kLimbs] = {}, lb[F::kLimbs] = {};
-    for (int i = 0; i < 8 && i < F::kLimbs; i++) {
-        la[i] = a[i];
-        lb[i] = b[i];
-    }
-    const fe x = raw ? F::load(la) : F::from_words(a), y = raw ? F::load(lb) : F::from_words(b);
+    const fe x = raw ? F::load(a) : F::from_words(a), y = raw ? F::load(b) : F::from_words(b);
     fe r;
     switch (op) {
         case kFeMul: r = F::mul(x, y); break;
@@ -56,12 +61,11 @@ LXV_HD void debug_fe_op(int op, const uint32_t a[8], const uint32_t b[8], uint32
         default: r = raw ? x : F::mul(x, F::one()); break;
     }
     if (raw && op != kFeCanon) {
-        uint32_t lo[F::kLimbs];
-        F::store(r, lo);
-        for (int i = 0; i < 8 && i < F::kLimbs; i++) out[i] = lo[i];
+        F::store(r, out);
         return;
     }
     F::to_words(r, out);
+    for (int i = 8; i < (raw ? F::kLimbs : 8); i++) out[i] = 0;
 }
 
 bool walk_supported(int layout, int batch, int waves);

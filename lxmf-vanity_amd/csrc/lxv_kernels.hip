@@ -84,13 +84,14 @@ template <class F>
 __global__ void debug_fe_kernel(int op, const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t wa[8], wb[8], wo[8];
-    for (int k = 0; k < 8; k++) {
-        wa[k] = a[i * 8 + k];
-        wb[k] = b[i * 8 + k];
+    const int w = debug_fe_stride<F>(op);
+    uint32_t wa[F::kLimbs > 8 ? F::kLimbs : 8], wb[F::kLimbs > 8 ? F::kLimbs : 8], wo[F::kLimbs > 8 ? F::kLimbs : 8];
+    for (int k = 0; k < w; k++) {
+        wa[k] = a[(size_t)i * w + k];
+        wb[k] = b[(size_t)i * w + k];
     }
     debug_fe_op<F>(op, wa, wb, wo);
-    for (int k = 0; k < 8; k++) out[i * 8 + k] = wo[k];
+    for (int k = 0; k < w; k++) out[(size_t)i * w + k] = wo[k];
 }
 
 __global__ void debug_addr_kernel(const ShaSearch c, const uint32_t* u, uint32_t* out, uint32_t n) {
@@ -111,9 +112,10 @@ __global__ void debug_addr_kernel(const ShaSearch c, const uint32_t* u, uint32_t
 // SIMDs actually ran at from the kernel itself.
 enum ValuProbeOp {
     kVpAlignbit = 0, kVpBitop3 = 1, kVpAdd3 = 2, kVpAdd = 3, kVpXor = 4, kVpPerm = 5, kVpShift = 6,
-    kVpMad64 = 7, kVpShaMix = 8, kVpAddCo = 9, kVpCarryChain = 10, kVpWalkMix = 11, kVpOps = 12
+    kVpMad64 = 7, kVpShaMix = 8, kVpAddCo = 9, kVpCarryChain = 10, kVpWalkMix = 11, kVpMulLo = 12,
+    kVpLshlAdd = 13, kVpLshlOr = 14, kVpOps = 15
 };
-constexpr int kValuOpsPerIter[kVpOps] = {32, 32, 32, 32, 32, 32, 32, 32, 56, 32, 36, 200};
+constexpr int kValuOpsPerIter[kVpOps] = {32, 32, 32, 32, 32, 32, 32, 32, 56, 32, 36, 200, 32, 32, 32};
 
 #define LXV_R8(I) I(0) I(1) I(2) I(3) I(4) I(5) I(6) I(7)
 #define LXV_R32(I) LXV_R8(I) LXV_R8(I) LXV_R8(I) LXV_R8(I)
@@ -175,6 +177,21 @@ __global__ __launch_bounds__(256) void valu_rate_kernel(uint32_t* out, unsigned 
                            [c0] "=&s"(c0_), [c1] "=&s"(c1_), [c2] "=&s"(c2_), [c3] "=&s"(c3_), [c4] "=&s"(c4_),
                            [c5] "=&s"(c5_), [c6] "=&s"(c6_), [c7] "=&s"(c7_)
                          : "v"(y), "v"(z));
+#undef LXV_I
+        } else if constexpr (kOp == kVpMulLo) {
+#define LXV_I(k) "v_mul_lo_u32 %" #k ", %" #k ", %8\n"
+            asm volatile(LXV_R32(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+                         "+v"(x7) : "v"(y), "v"(z));
+#undef LXV_I
+        } else if constexpr (kOp == kVpLshlAdd) {
+#define LXV_I(k) "v_lshl_add_u32 %" #k ", %" #k ", 4, %8\n"
+            asm volatile(LXV_R32(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+                         "+v"(x7) : "v"(y), "v"(z));
+#undef LXV_I
+        } else if constexpr (kOp == kVpLshlOr) {
+#define LXV_I(k) "v_lshl_or_b32 %" #k ", %" #k ", 6, %8\n"
+            asm volatile(LXV_R32(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
+                         "+v"(x7) : "v"(y), "v"(z));
 #undef LXV_I
         } else if constexpr (kOp == kVpShaMix) {
             // per chain: 3 rotates, 2 three-input boolean functions, 2
@@ -423,7 +440,8 @@ hipError_t launch_valu_rate(int op, uint32_t* out, unsigned long long* clocks, u
     static const probe_fn fns[kVpOps] = {
         valu_rate_kernel<0>, valu_rate_kernel<1>, valu_rate_kernel<2>, valu_rate_kernel<3>,
         valu_rate_kernel<4>, valu_rate_kernel<5>, valu_rate_kernel<6>, valu_rate_kernel<7>,
-        valu_rate_kernel<8>, valu_rate_kernel<9>, valu_rate_kernel<10>, valu_rate_kernel<11>};
+        valu_rate_kernel<8>, valu_rate_kernel<9>, valu_rate_kernel<10>, valu_rate_kernel<11>,
+        valu_rate_kernel<12>, valu_rate_kernel<13>, valu_rate_kernel<14>};
     if (op < 0 || op >= kVpOps) return hipErrorInvalidValue;
     hipLaunchKernelGGL(fns[op], dim3(blocks), dim3(256), 0, s, out, clocks, seed, iters);
     return hipGetLastError();

@@ -28,6 +28,7 @@
 // two keys per pair.
 #pragma once
 
+#include "fe10.h"
 #include "lxv_walk.h"
 #include "pair_table.inc"
 
@@ -41,16 +42,50 @@ static const uint32_t kPairTab[LXV_PAIR_TABLE_ENTRIES * 16] = LXV_PAIR_TABLE_INI
 
 constexpr uint32_t kMontA = 486662;  // Curve25519: v^2 = u^3 + A u^2 + u
 
+// The same table in the limbs of the 10 x 25.5 layout (20 words per entry: u,
+// then v), converted at compile time, so that the walk loads the limbs as
+// they are instead of converting words on every use.
+struct PairTab10 {
+    uint32_t w[LXV_PAIR_TABLE_ENTRIES * 20];
+};
+
+constexpr PairTab10 make_pair_tab10() {
+    constexpr uint32_t src[LXV_PAIR_TABLE_ENTRIES * 16] = LXV_PAIR_TABLE_INIT;
+    PairTab10 t{};
+    for (int e = 0; e < LXV_PAIR_TABLE_ENTRIES; e++)
+        for (int h = 0; h < 2; h++)
+            for (int i = 0; i < 10; i++) {
+                const int bit = fe10_bit(i), word = bit >> 5, shift = bit & 31;
+                const uint32_t lo = src[e * 16 + h * 8 + word];
+                const uint32_t hi = word + 1 < 8 ? src[e * 16 + h * 8 + word + 1] : 0u;
+                const uint32_t v = shift ? (lo >> shift) | (hi << (32 - shift)) : lo;
+                t.w[e * 20 + h * 10 + i] = v & fe10_mask(i);
+            }
+    return t;
+}
+
+#if defined(__HIP_DEVICE_COMPILE__)
+static __constant__ PairTab10 kPairTab10 = make_pair_tab10();
+#else
+static const PairTab10 kPairTab10 = make_pair_tab10();
+#endif
+
 LXV_HD int pair_tab_entry(int i) { return i <= 128 ? i - 1 : 128; }
 
 // u and v of T_i (1 <= i <= 128, or i = 256) in the layout of F.
 template <class F>
 LXV_HD typename F::fe pair_tab_u(int i) {
-    return F::from_words(kPairTab + 16 * pair_tab_entry(i));
+    if constexpr (F::kLimbs == 10)
+        return F::load(kPairTab10.w + 20 * pair_tab_entry(i));
+    else
+        return F::from_words(kPairTab + 16 * pair_tab_entry(i));
 }
 template <class F>
 LXV_HD typename F::fe pair_tab_v(int i) {
-    return F::from_words(kPairTab + 16 * pair_tab_entry(i) + 8);
+    if constexpr (F::kLimbs == 10)
+        return F::load(kPairTab10.w + 20 * pair_tab_entry(i) + 10);
+    else
+        return F::from_words(kPairTab + 16 * pair_tab_entry(i) + 8);
 }
 
 template <class F>
@@ -61,8 +96,9 @@ LXV_HD typename F::fe fe_small(uint32_t k) {
 
 // One batch of B keys around the centre (xc, yc), which moves on to the next
 // batch's centre. emit(i, x) receives the public key of key n + i (i < B) as
-// a field element; mem is the lane's prefix-product array (B/2 elements,
-// lane-interleaved with stride lanes).
+// a field element, not necessarily tight (for encoding only); mem is the
+// lane's prefix-product array (B/2 elements, lane-interleaved with stride
+// lanes).
 template <class F, int B, class Emit>
 LXV_HD void pair_batch(typename F::fe& xc, typename F::fe& yc, uint2* mem, uint32_t lanes, uint32_t lane,
                        Emit&& emit) {
@@ -82,33 +118,32 @@ LXV_HD void pair_batch(typename F::fe& xc, typename F::fe& yc, uint2* mem, uint3
     fe inv = F::invert(prod);
     const fe sa = F::tight(F::add(xc, fe_small<F>(kMontA)));
 
-    // backward, one key per step: j = 0 the single key (e = H + 1), then
-    // e = H .. 1 with C + T_e on odd j and C - T_e on even j, then the
-    // centre's own key. The element's inverse is formed on the first key.
+    // backward, by denominator: e = H + 1 (its single key C + T_{H+1}), then
+    // e = H .. 1 (the pair C + T_e, C - T_e sharing its inverse), then e = 0
+    // for the centre's own key (the centre step's inverse is left in inv).
+    // Every key is x = (ie (v_e - y'))^2 - s with y' = y or -y: for e = 0,
+    // ie = 0 and s = -x(C) give x(C) itself. The keys of one e run in an
+    // inner loop that is not unrolled, so a single copy of the hash follows
+    // and the inverse, s and the running product stay put in registers.
+    const fe nyc = F::tight(F::sub(F::zero(), yc));
     fe ie, s;
-    int e = H + 1;
-    for (int j = 0; j < B; j++) {
-        fe x;
-        int i;
-        if (j == B - 1) {
-            x = xc;
-            i = H;
-        } else if (j == 0 || (j & 1)) {
-            e = j == 0 ? H + 1 : H - (j - 1) / 2;
+    for (int e = H + 1; e >= 0; e--) {
+        if (e > 0) {
             const fe ue = pair_tab_u<F>(e);
-            const fe ve = pair_tab_v<F>(e);
             ie = F::mul(inv, load_fe<F>(mem + (size_t)(e - 1) * slot, lanes, lane));
             inv = F::mul(inv, F::sub(ue, xc));
             s = F::tight(F::add(sa, ue));
-            x = F::tight(F::sub(F::sq(F::mul(F::sub(ve, yc), ie)), s));
-            i = H + e;
         } else {
-            // v(T_e) comes again from the table (scalar loads) rather than
-            // across the hash of the first key, which frees 8 VGPRs
-            x = F::tight(F::sub(F::sq(F::mul(F::add(pair_tab_v<F>(e), yc), ie)), s));
-            i = H - e;
+            ie = F::zero();
+            s = F::tight(F::sub(F::zero(), xc));
         }
-        emit(i, x);
+        const int keys = (e == H + 1 || e == 0) ? 1 : 2;
+#pragma clang loop unroll(disable)
+        for (int h = 0; h < keys; h++) {
+            const fe yh = h == 0 ? yc : nyc;
+            const fe x = F::sub(F::sq(F::mul(F::sub(pair_tab_v<F>(e > 0 ? e : 1), yh), ie)), s);
+            emit(e == 0 ? H : h == 0 ? H + e : H - e, x);
+        }
     }
 
     // the next centre, C + T_B: inv is now 1 / d_0
@@ -147,7 +182,7 @@ template <class F>
 LXV_HD void affine_add(typename F::fe& x, typename F::fe& y, const typename F::fe& ut, const typename F::fe& vt) {
     typedef typename F::fe fe;
     const fe l = F::mul(F::sub(vt, y), F::invert(F::sub(ut, x)));
-    const fe x2 = F::tight(F::sub(F::sub(F::sq(l), fe_small<F>(kMontA)), F::add(x, ut)));
+    const fe x2 = F::tight(F::sub(F::sub(F::sub(F::sq(l), fe_small<F>(kMontA)), x), ut));
     y = F::tight(F::sub(F::mul(l, F::sub(x, x2)), y));
     x = x2;
 }

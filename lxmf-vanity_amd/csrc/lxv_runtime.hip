@@ -654,19 +654,19 @@ extern "C" int lxv_debug_fe(int layout, int op, const uint32_t* a, const uint32_
                             int on_gpu) {
     if (!a || !b || !out || (op & 0xff) > 7 || (op & ~(0xff | LXV_FE_RAW_LIMBS)) != 0) return LXV_E_ARG;
     if (layout != LXV_LAYOUT_10X26 && layout != LXV_LAYOUT_8X32) return LXV_E_ARG;
-    if ((op & LXV_FE_RAW_LIMBS) && layout != LXV_LAYOUT_8X32) return LXV_E_ARG;
+    const size_t w = layout == LXV_LAYOUT_8X32 ? debug_fe_stride<Field8>(op) : debug_fe_stride<Field10>(op);
     if (!on_gpu) {
         for (uint32_t i = 0; i < n; i++) {
             if (layout == LXV_LAYOUT_8X32)
-                debug_fe_op<Field8>(op, a + 8 * i, b + 8 * i, out + 8 * i);
+                debug_fe_op<Field8>(op, a + w * i, b + w * i, out + w * i);
             else
-                debug_fe_op<Field10>(op, a + 8 * i, b + 8 * i, out + 8 * i);
+                debug_fe_op<Field10>(op, a + w * i, b + w * i, out + w * i);
         }
         return LXV_OK;
     }
     if (n == 0) return LXV_OK;
     uint32_t *da = nullptr, *db = nullptr, *dout = nullptr;
-    const size_t bytes = (size_t)n * 8 * sizeof(uint32_t);
+    const size_t bytes = (size_t)n * w * sizeof(uint32_t);
     hipError_t e = hipSetDevice(0);
     if (e == hipSuccess) e = hipMalloc((void**)&da, bytes);
     if (e == hipSuccess) e = hipMalloc((void**)&db, bytes);

@@ -29,7 +29,7 @@ LXV_OK = 0
 LXV_E_BUSY = -5
 LAYOUT_8X32 = 0
 LAYOUT_10X26 = 1
-FE_RAW_LIMBS = 0x100            # lxv_debug_fe: raw 8 x 32 limbs in and out (bit 255 kept)
+FE_RAW_LIMBS = 0x100            # lxv_debug_fe: the layout's raw limbs in and out
 WALK_PAIRS = 0                  # batch keys around an affine centre (the default)
 WALK_PROJECTIVE = 1             # the reference's chained x-only differential additions
 MAX_HITS = 4096                 # hits recorded per launch (src/gpu/mod.rs:23)
@@ -488,14 +488,18 @@ class Multi:
 
 
 def debug_fe(op, a_words, b_words, on_gpu, layout=LAYOUT_8X32):
-    n = len(a_words) // 8
+    """op(a, b) per element by the device field code (on the GPU or its host
+    build). Elements are 8 words each, or with FE_RAW_LIMBS the layout's raw
+    limbs (8 words for 8x32, 10 for 10x26); so is the result."""
+    w = 10 if (op & FE_RAW_LIMBS) and layout == LAYOUT_10X26 else 8
+    n = len(a_words) // w
     A = (ctypes.c_uint32 * max(len(a_words), 1))(*a_words)
     B = (ctypes.c_uint32 * max(len(b_words), 1))(*b_words)
-    O = (ctypes.c_uint32 * max(8 * n, 1))()
+    O = (ctypes.c_uint32 * max(w * n, 1))()
     rc = lib().lxv_debug_fe(layout, op, A, B, O, n, 1 if on_gpu else 0)
     if rc != LXV_OK:
         raise RuntimeError("lxv_debug_fe failed (%d)" % rc)
-    return list(O[:8 * n])
+    return list(O[:w * n])
 
 
 def tuned_shape(workload):
@@ -531,7 +535,8 @@ VALU_PROBES = {0: "v_alignbit_b32", 1: "v_bitop3_b32", 2: "v_add3_u32", 3: "v_ad
                5: "v_perm_b32", 6: "v_lshrrev_b32", 7: "v_mad_u64_u32",
                8: "SHA-256 round mix (3 alignbit : 2 bitop3 : 2 add3)", 9: "v_add_co_u32",
                10: "v_addc_co_u32 dependent carry chain",
-               11: "pair-walk mix (30% alignbit, 19% bitop3, 12% add3, 9% addc, 8% mad64, 8% add, 6% mov, 3% shr)"}
+               11: "pair-walk mix (30% alignbit, 19% bitop3, 12% add3, 9% addc, 8% mad64, 8% add, 6% mov, 3% shr)",
+               12: "v_mul_lo_u32", 13: "v_lshl_add_u32", 14: "v_lshl_or_b32"}
 
 
 def debug_address(ed_words, nh_words, u_words, on_gpu):

@@ -57,6 +57,23 @@ def test_gpu_field_ops_match_big_integers(layout):
         assert got == want, op
 
 
+@pytest.mark.parametrize("op", [0, 1, 2, 3, 5, 6, 4])
+def test_gpu_fe10_raw_limbs_at_their_bounds(op):
+    # The 10 x 25.5 code on the GPU (column scans on v_mad_u64_u32) at the top
+    # of each operation's input range: results congruent mod p and within
+    # their output bounds, and limb for limb equal to the host build.
+    import fe10_edge
+    pairs = fe10_edge.cases(op)
+    if op == 4:
+        pairs = pairs[::3]
+    A, B = fe10_edge.flat(pairs)
+    print("fe10 raw op", op, "cases", len(pairs), flush=True)
+    out = lxv.debug_fe(op | lxv.FE_RAW_LIMBS, A, B, on_gpu=True, layout=lxv.LAYOUT_10X26)
+    assert fe10_edge.check(op, pairs, out) == []
+    host = lxv.debug_fe(op | lxv.FE_RAW_LIMBS, A, B, on_gpu=False, layout=lxv.LAYOUT_10X26)
+    assert out == host
+
+
 @pytest.mark.parametrize("op", [0, 1, 2, 3, 5, 6, 4, 7])
 def test_gpu_fe8_raw_limbs_weakly_reduced_inputs(op):
     # The 8 x 32 asm on its own working range: raw limbs in [0, 2^256) with
@@ -389,7 +406,7 @@ def test_headline_shape_2p28_subrange_matches_oracle():
         b.close()
 
 
-@pytest.mark.parametrize("op", range(12))
+@pytest.mark.parametrize("op", range(15))
 def test_valu_probe_rates_are_sane(op):
     # The issue-rate probes the VALU roofline is priced against: each runs,
     # reports a positive rate no more than twice one wave64 instruction per

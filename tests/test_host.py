@@ -244,6 +244,49 @@ def test_fe8_raw_limbs_on_host(op):
     assert fe_edge.check(op, pairs, out, raw=True) == []
 
 
+@pytest.mark.parametrize("op", [0, 1, 2, 3, 5, 6, 4])
+def test_fe10_raw_limbs_on_host(op):
+    # The host build of the 10 x 25.5 field code at the top of each
+    # operation's input range (run-over limbs), outputs congruent mod p and
+    # within their promised bounds.
+    import fe10_edge
+    pairs = fe10_edge.cases(op)
+    if op == 4:
+        pairs = pairs[::3]
+    A, B = fe10_edge.flat(pairs)
+    out = lxv.debug_fe(op | lxv.FE_RAW_LIMBS, A, B, on_gpu=False, layout=lxv.LAYOUT_10X26)
+    assert fe10_edge.check(op, pairs, out) == []
+
+
+def test_fe10_column_sums_fit_64_bits():
+    # fe10.h's bound: with loose inputs every pre-scaled operand (2 f_i,
+    # 4 f_i, 19 g_j) fits 32 bits and every column sum, with the largest
+    # carry in, fits 64 bits; the carry out of a column fits two words.
+    import fe10_edge as fe
+    L = fe.LOOSE_MAX
+    carry_in = 2**39
+    worst = 0
+    for k in range(10):
+        s_mul, s_sq = carry_in, carry_in
+        for i in range(10):
+            j = (k - i) % 10
+            a = L[i] * (2 if (i % 2 and j % 2) else 1)
+            b = L[j] * (19 if i + j >= 10 else 1)
+            assert a < 2**32 and b < 2**32, (k, i, j)
+            s_mul += a * b
+            if j >= i:
+                dbl = (1 if i < j else 0) + (1 if (i % 2 and j % 2) else 0)
+                a2 = L[i] << dbl
+                assert a2 < 2**32, (k, i, j)
+                s_sq += a2 * b
+        worst = max(worst, s_mul, s_sq)
+    assert worst < 2**63, worst.bit_length()
+    assert worst >> 25 < carry_in
+    # and the bound is not vacuous: doubling a wrapped even operand (the
+    # doubling kept on the b side) would not fit 32 bits
+    assert L[8] * 19 * 2 >= 2**32
+
+
 def test_device_address_hash_on_host():
     rng = random.Random(25)
     ed_pub = bytes(rng.getrandbits(8) for _ in range(32))
