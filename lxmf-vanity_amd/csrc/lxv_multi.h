@@ -92,11 +92,20 @@ LXV_D uint32_t wave_uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(
 // lowers the search's best key of this step and its lowest-hit block, with
 // two atomics per wave with matches and nothing stored per hit, so the work
 // of a step does not grow with its matches.
-LXV_D void multi_emit(const MultiParams& p, const SlotConst& sc, uint32_t slot, uint32_t v, uint32_t blk, uint64_t n,
+//
+// The slot's hash constants are re-read (scalar loads) for every key rather
+// than held in SGPRs across the unit's walk, which would spill them to VGPR
+// lanes (see walk_params_reloaded); the slot table is not written while a
+// step runs, so it is read through the constant address space.
+LXV_D void multi_emit(const MultiParams& p, uint32_t slot, uint32_t v, uint32_t blk, uint64_t n,
                       const uint32_t uw[8]) {
+    typedef __attribute__((address_space(4))) const SlotConst* slot_cptr;
+    slot_cptr sp = (slot_cptr)(p.slots + slot);
+    asm volatile("" : "+s"(sp));
+    const ShaSearch& sha = ((const SlotConst*)sp)->sha;
     uint32_t addr[4];
-    lxmf_address_words(uw, sc.sha, addr);
-    const uint64_t ball = __ballot(prefix_match(addr, sc.sha));
+    lxmf_address_words(uw, sha, addr);
+    const uint64_t ball = __ballot(prefix_match(addr, sha));
     if (ball && __lane_id() == (uint32_t)__builtin_ctzll(ball)) {
         atomicMin(&p.best[slot], hit_key(n, v));
         atomicMin(&p.dyn[slot].found_block, blk);
@@ -179,7 +188,7 @@ LXV_D void multi_wave(const MultiParams& p, uint32_t plane) {
                     if (i > 0) inv = F::mul(inv, load_fe<F>(bs + (size_t)kPairs * PL, PL, plane));
                     uint32_t uw[8];
                     F::to_words(u_, uw);
-                    multi_emit(p, sc, slot, v, blk, n0 + (uint64_t)i, uw);
+                    multi_emit(p, slot, v, blk, n0 + (uint64_t)i, uw);
                 }
                 keys += B;
             }
@@ -211,7 +220,7 @@ LXV_D void multi_pair_wave(const MultiParams& p, uint32_t plane) {
                 pair_batch<F, B>(xc, yc, p.batch_mem, PL, plane, [&](int i, const fe& x) {
                     uint32_t uw[8];
                     F::to_words(x, uw);
-                    multi_emit(p, sc, slot, v, blk, n0 + (uint64_t)i, uw);
+                    multi_emit(p, slot, v, blk, n0 + (uint64_t)i, uw);
                 });
                 keys += B;
             }

@@ -259,6 +259,20 @@ LXV_HD void pair_centre_offset(const PairAnchor& p0, const PairAnchor& s, uint64
 }
 
 #if defined(__HIPCC__)
+// The launch's parameters, re-read from the kernel argument segment (scalar
+// loads) where a key uses them, rather than held in SGPRs across the whole
+// walk: with the hash's per-search constants (45 words) live throughout, the
+// SGPRs ran out, the compiler spilled them to VGPR lanes, and every key paid
+// about 60 v_readlane to fetch them back. The empty asm makes the pointer
+// opaque, so the loads stay next to their use. Only for kernels whose one
+// argument is the WalkParams (it then sits at the segment's start).
+__device__ __forceinline__ const WalkParams& walk_params_reloaded() {
+    typedef __attribute__((address_space(4))) const WalkParams* kernarg_t;
+    kernarg_t kp = (kernarg_t)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(kp));
+    return *(const WalkParams*)kp;
+}
+
 // The walk of one lane: `iters` batches of B keys. The centre is the lane's
 // whole state (state slots 0 and 1: x and y); the batch arrays are those of
 // lane mem_lane of mem_lanes.
@@ -288,13 +302,14 @@ __device__ __forceinline__ void walk_pair_lane(const WalkParams& p, uint32_t lan
 #pragma unroll
                 for (int k = 0; k < 8; k++) d[k] = uw[k];
             }
+            const WalkParams& kp = walk_params_reloaded();
             uint32_t addr[4];
-            lxmf_address_words(uw, p.sha, addr);
+            lxmf_address_words(uw, kp.sha, addr);
             x0 ^= addr[0];
             x1 ^= addr[1];
             x2 ^= addr[2];
             x3 ^= addr[3];
-            append_hit(prefix_match(addr, p.sha), lane, n + (uint64_t)i, addr, p.hits, p.hit_count, p.max_hits);
+            append_hit(prefix_match(addr, kp.sha), lane, n + (uint64_t)i, addr, kp.hits, kp.hit_count, kp.max_hits);
         });
         n += B;
     }
