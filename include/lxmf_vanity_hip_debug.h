@@ -63,6 +63,7 @@ int lxv_debug_address(const uint32_t ed[8], const uint32_t nh[3], const uint32_t
 #define LXV_VALU_MUL_LO 12      /* v_mul_lo_u32 */
 #define LXV_VALU_LSHL_ADD 13    /* v_lshl_add_u32 (shift and add, as in x * 19 = (x << 4) + 3x) */
 #define LXV_VALU_LSHL_OR 14     /* v_lshl_or_b32 (packing limbs into words) */
+#define LXV_VALU_MAD64_CHAIN 15 /* v_mad_u64_u32 in one dependent chain per lane (a product column) */
 
 /* Measures the issue rate of probe `op` on `device` with one resident round
  * of waves_per_cu waves per CU (a multiple of 4, at most 32): lane-level

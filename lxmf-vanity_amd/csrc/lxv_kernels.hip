@@ -113,9 +113,9 @@ __global__ void debug_addr_kernel(const ShaSearch c, const uint32_t* u, uint32_t
 enum ValuProbeOp {
     kVpAlignbit = 0, kVpBitop3 = 1, kVpAdd3 = 2, kVpAdd = 3, kVpXor = 4, kVpPerm = 5, kVpShift = 6,
     kVpMad64 = 7, kVpShaMix = 8, kVpAddCo = 9, kVpCarryChain = 10, kVpWalkMix = 11, kVpMulLo = 12,
-    kVpLshlAdd = 13, kVpLshlOr = 14, kVpOps = 15
+    kVpLshlAdd = 13, kVpLshlOr = 14, kVpMad64Chain = 15, kVpOps = 16
 };
-constexpr int kValuOpsPerIter[kVpOps] = {32, 32, 32, 32, 32, 32, 32, 32, 56, 32, 36, 200, 32, 32, 32};
+constexpr int kValuOpsPerIter[kVpOps] = {32, 32, 32, 32, 32, 32, 32, 32, 56, 32, 36, 200, 32, 32, 32, 32};
 
 #define LXV_R8(I) I(0) I(1) I(2) I(3) I(4) I(5) I(6) I(7)
 #define LXV_R32(I) LXV_R8(I) LXV_R8(I) LXV_R8(I) LXV_R8(I)
@@ -192,6 +192,13 @@ __global__ __launch_bounds__(256) void valu_rate_kernel(uint32_t* out, unsigned 
 #define LXV_I(k) "v_lshl_or_b32 %" #k ", %" #k ", 6, %8\n"
             asm volatile(LXV_R32(LXV_I) : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3), "+v"(x4), "+v"(x5), "+v"(x6),
                          "+v"(x7) : "v"(y), "v"(z));
+#undef LXV_I
+        } else if constexpr (kOp == kVpMad64Chain) {
+            // one dependent chain per lane: each v_mad_u64_u32 adds into the
+            // previous one's result (a column of a multiplication)
+            uint64_t cj_;
+#define LXV_I(k) "v_mad_u64_u32 %0, %1, %2, %3, %0\n"
+            asm volatile(LXV_R32(LXV_I) : "+v"(w0), "=&s"(cj_) : "v"(y), "v"(z));
 #undef LXV_I
         } else if constexpr (kOp == kVpShaMix) {
             // per chain: 3 rotates, 2 three-input boolean functions, 2
@@ -441,7 +448,7 @@ hipError_t launch_valu_rate(int op, uint32_t* out, unsigned long long* clocks, u
         valu_rate_kernel<0>, valu_rate_kernel<1>, valu_rate_kernel<2>, valu_rate_kernel<3>,
         valu_rate_kernel<4>, valu_rate_kernel<5>, valu_rate_kernel<6>, valu_rate_kernel<7>,
         valu_rate_kernel<8>, valu_rate_kernel<9>, valu_rate_kernel<10>, valu_rate_kernel<11>,
-        valu_rate_kernel<12>, valu_rate_kernel<13>, valu_rate_kernel<14>};
+        valu_rate_kernel<12>, valu_rate_kernel<13>, valu_rate_kernel<14>, valu_rate_kernel<15>};
     if (op < 0 || op >= kVpOps) return hipErrorInvalidValue;
     hipLaunchKernelGGL(fns[op], dim3(blocks), dim3(256), 0, s, out, clocks, seed, iters);
     return hipGetLastError();

@@ -536,7 +536,8 @@ VALU_PROBES = {0: "v_alignbit_b32", 1: "v_bitop3_b32", 2: "v_add3_u32", 3: "v_ad
                8: "SHA-256 round mix (3 alignbit : 2 bitop3 : 2 add3)", 9: "v_add_co_u32",
                10: "v_addc_co_u32 dependent carry chain",
                11: "pair-walk mix (30% alignbit, 19% bitop3, 12% add3, 9% addc, 8% mad64, 8% add, 6% mov, 3% shr)",
-               12: "v_mul_lo_u32", 13: "v_lshl_add_u32", 14: "v_lshl_or_b32"}
+               12: "v_mul_lo_u32", 13: "v_lshl_add_u32", 14: "v_lshl_or_b32",
+               15: "v_mad_u64_u32 dependent chain"}
 
 
 def debug_address(ed_words, nh_words, u_words, on_gpu):

@@ -406,7 +406,7 @@ def test_headline_shape_2p28_subrange_matches_oracle():
         b.close()
 
 
-@pytest.mark.parametrize("op", range(15))
+@pytest.mark.parametrize("op", range(16))
 def test_valu_probe_rates_are_sane(op):
     # The issue-rate probes the VALU roofline is priced against: each runs,
     # reports a positive rate no more than twice one wave64 instruction per
