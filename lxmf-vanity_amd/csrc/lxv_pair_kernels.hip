@@ -20,6 +20,9 @@
 
 namespace lxv {
 
+// Both walk kernels take the WalkParams as their only argument: walk_pair_lane
+// re-reads the per-search hash constants from the start of the kernel
+// argument segment (walk_params_reloaded).
 template <class F, int B, int W>
 __global__ __launch_bounds__(64, W) void walk_pair_kernel(const WalkParams p) {
     const uint32_t lane = blockIdx.x * 64u + threadIdx.x;
