@@ -42,31 +42,62 @@ static const uint32_t kPairTab[LXV_PAIR_TABLE_ENTRIES * 16] = LXV_PAIR_TABLE_INI
 
 constexpr uint32_t kMontA = 486662;  // Curve25519: v^2 = u^3 + A u^2 + u
 
-// The same table in the limbs of the 10 x 25.5 layout (20 words per entry: u,
-// then v), converted at compile time, so that the walk loads the limbs as
-// they are instead of converting words on every use.
+// -v(T_i) = p - v(T_i) as canonical little-endian words (v is never 0: no
+// table point has order 2). The second key of a pair, x(C - T_i), takes its
+// slope from (-v - y) / (u - x), the negative of (v + y) / (u - x), whose
+// square is the same: both keys of a pair are then (v' - y) times the pair's
+// inverse, with v' picked from one table or the other by a uniform index.
+struct PairTabNV {
+    uint32_t w[LXV_PAIR_TABLE_ENTRIES * 8];
+};
+
+constexpr PairTabNV make_pair_tab_nv() {
+    constexpr uint32_t src[LXV_PAIR_TABLE_ENTRIES * 16] = LXV_PAIR_TABLE_INIT;
+    // p = 2^255 - 19
+    constexpr uint32_t p[8] = {0xffffffedu, 0xffffffffu, 0xffffffffu, 0xffffffffu,
+                               0xffffffffu, 0xffffffffu, 0xffffffffu, 0x7fffffffu};
+    PairTabNV t{};
+    for (int e = 0; e < LXV_PAIR_TABLE_ENTRIES; e++) {
+        int64_t b = 0;
+        for (int i = 0; i < 8; i++) {
+            const int64_t d = (int64_t)p[i] - src[e * 16 + 8 + i] + b;
+            t.w[e * 8 + i] = (uint32_t)d;
+            b = d < 0 ? -1 : 0;
+        }
+    }
+    return t;
+}
+
+// The table in the limbs of the 10 x 25.5 layout (30 words per entry: u, v,
+// -v), converted at compile time, so that the walk loads the limbs as they
+// are instead of converting words on every use.
 struct PairTab10 {
-    uint32_t w[LXV_PAIR_TABLE_ENTRIES * 20];
+    uint32_t w[LXV_PAIR_TABLE_ENTRIES * 30];
 };
 
 constexpr PairTab10 make_pair_tab10() {
     constexpr uint32_t src[LXV_PAIR_TABLE_ENTRIES * 16] = LXV_PAIR_TABLE_INIT;
+    constexpr PairTabNV nv = make_pair_tab_nv();
     PairTab10 t{};
     for (int e = 0; e < LXV_PAIR_TABLE_ENTRIES; e++)
-        for (int h = 0; h < 2; h++)
+        for (int h = 0; h < 3; h++) {
+            const uint32_t* words = h < 2 ? src + e * 16 + h * 8 : nv.w + e * 8;
             for (int i = 0; i < 10; i++) {
                 const int bit = fe10_bit(i), word = bit >> 5, shift = bit & 31;
-                const uint32_t lo = src[e * 16 + h * 8 + word];
-                const uint32_t hi = word + 1 < 8 ? src[e * 16 + h * 8 + word + 1] : 0u;
+                const uint32_t lo = words[word];
+                const uint32_t hi = word + 1 < 8 ? words[word + 1] : 0u;
                 const uint32_t v = shift ? (lo >> shift) | (hi << (32 - shift)) : lo;
-                t.w[e * 20 + h * 10 + i] = v & fe10_mask(i);
+                t.w[e * 30 + h * 10 + i] = v & fe10_mask(i);
             }
+        }
     return t;
 }
 
 #if defined(__HIP_DEVICE_COMPILE__)
+static __constant__ PairTabNV kPairTabNV = make_pair_tab_nv();
 static __constant__ PairTab10 kPairTab10 = make_pair_tab10();
 #else
+static const PairTabNV kPairTabNV = make_pair_tab_nv();
 static const PairTab10 kPairTab10 = make_pair_tab10();
 #endif
 
@@ -76,16 +107,24 @@ LXV_HD int pair_tab_entry(int i) { return i <= 128 ? i - 1 : 128; }
 template <class F>
 LXV_HD typename F::fe pair_tab_u(int i) {
     if constexpr (F::kLimbs == 10)
-        return F::load(kPairTab10.w + 20 * pair_tab_entry(i));
+        return F::load(kPairTab10.w + 30 * pair_tab_entry(i));
     else
         return F::from_words(kPairTab + 16 * pair_tab_entry(i));
 }
 template <class F>
 LXV_HD typename F::fe pair_tab_v(int i) {
     if constexpr (F::kLimbs == 10)
-        return F::load(kPairTab10.w + 20 * pair_tab_entry(i) + 10);
+        return F::load(kPairTab10.w + 30 * pair_tab_entry(i) + 10);
     else
         return F::from_words(kPairTab + 16 * pair_tab_entry(i) + 8);
+}
+// v(T_i), or -v(T_i) when neg (a uniform choice between two tables).
+template <class F>
+LXV_HD typename F::fe pair_tab_v_signed(int i, bool neg) {
+    if constexpr (F::kLimbs == 10)
+        return F::load(kPairTab10.w + 30 * pair_tab_entry(i) + (neg ? 20 : 10));
+    else
+        return F::from_words(neg ? kPairTabNV.w + 8 * pair_tab_entry(i) : kPairTab + 16 * pair_tab_entry(i) + 8);
 }
 
 template <class F>
@@ -121,11 +160,10 @@ LXV_HD void pair_batch(typename F::fe& xc, typename F::fe& yc, uint2* mem, uint3
     // backward, by denominator: e = H + 1 (its single key C + T_{H+1}), then
     // e = H .. 1 (the pair C + T_e, C - T_e sharing its inverse), then e = 0
     // for the centre's own key (the centre step's inverse is left in inv).
-    // Every key is x = (ie (v_e - y'))^2 - s with y' = y or -y: for e = 0,
+    // Every key is x = (ie (v' - y))^2 - s with v' = v_e or -v_e: for e = 0,
     // ie = 0 and s = -x(C) give x(C) itself. The keys of one e run in an
     // inner loop that is not unrolled, so a single copy of the hash follows
     // and the inverse, s and the running product stay put in registers.
-    const fe nyc = F::tight(F::sub(F::zero(), yc));
     fe ie, s;
     for (int e = H + 1; e >= 0; e--) {
         if (e > 0) {
@@ -140,8 +178,7 @@ LXV_HD void pair_batch(typename F::fe& xc, typename F::fe& yc, uint2* mem, uint3
         const int keys = (e == H + 1 || e == 0) ? 1 : 2;
 #pragma clang loop unroll(disable)
         for (int h = 0; h < keys; h++) {
-            const fe yh = h == 0 ? yc : nyc;
-            const fe x = F::sub(F::sq(F::mul(F::sub(pair_tab_v<F>(e > 0 ? e : 1), yh), ie)), s);
+            const fe x = F::sub(F::sq(F::mul(F::sub(pair_tab_v_signed<F>(e > 0 ? e : 1, h != 0), yc), ie)), s);
             emit(e == 0 ? H : h == 0 ? H + e : H - e, x);
         }
     }
