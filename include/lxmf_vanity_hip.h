@@ -303,7 +303,8 @@ typedef struct lxv_multi_opts {
     uint32_t no_graph;      /* 1: launch eagerly instead of replaying a hipGraph */
     uint64_t pool_bytes;    /* device bytes for resident walk state; 0 = 2 GiB */
     uint32_t max_vlanes;    /* cap on a search's virtual lanes; 0 = 262144 */
-    uint32_t lane_keys_log2;/* expected keys per virtual lane, log2 (8..20); 0 = 14.
+    uint32_t lane_keys_log2;/* expected keys per virtual lane, log2 (8..20); 0 = the
+                               tuned pick's (LXV_TUNED_BATCHED), else 14.
                                Smaller: more lanes per search, so a short search
                                finishes in fewer sequential blocks, at the cost of
                                one X25519 ladder per extra lane */

@@ -203,20 +203,26 @@ MIX_SHAPES = ([dict(walk=lxv.WALK_PAIRS, layout=lxv.LAYOUT_8X32, batch=b, waves_
                for b in (64, 128, 256) for w in (8, 16)] +
               [dict(walk=lxv.WALK_PAIRS, layout=lxv.LAYOUT_10X26, batch=256, waves_per_cu=w) for w in (8, 16)])
 SERVING_STEPS_MS = (15, 25, 50)
+# expected keys per virtual lane (log2) for the batched mix: fewer keys per
+# lane means more lanes per search and shorter units of work
+BATCHED_LANE_KEYS_LOG2 = (10, 12, 14)
 
 
 def mix_sweep(kind, wall_s, progress=print):
     """Runs the batched mix or the serving stream at every batched shape (and,
-    for serving, every step budget) for wall_s seconds each."""
+    for serving, every step budget; for the batched mix, every expected keys
+    per virtual lane) for wall_s seconds each."""
     import workloads
     points = []
     for sh in MIX_SHAPES:
-        steps = SERVING_STEPS_MS if kind == "serving" else (0,)
-        for step_ms in steps:
+        variants = SERVING_STEPS_MS if kind == "serving" else BATCHED_LANE_KEYS_LOG2
+        for v in variants:
             shape = {"batch": sh["batch"], "waves_per_cu": sh["waves_per_cu"], "layout": sh["layout"],
                      "walk": sh["walk"]}
+            step_ms = v if kind == "serving" else 0
             t0 = time.time()
             if kind == "batched":
+                shape["lane_keys_log2"] = v
                 r = workloads.batched(wall_s=wall_s, shape=shape, progress=None)
                 pt = {"keys_per_s": r["keys_per_s"], "found": r["found"]}
             else:
@@ -229,8 +235,9 @@ def mix_sweep(kind, wall_s, progress=print):
                        "step_ms": r["shape"]["step_ms"], "lane_keys_log2": r["shape"]["lane_keys_log2"]})
             points.append(pt)
             if progress:
-                progress("%s %-7s B=%-3d waves/CU=%-2d step %3d ms: %.3e keys/s%s  (%.0f s)" % (
-                    kind, pt["layout"], pt["batch"], pt["waves_per_cu"], pt["step_ms"], pt["keys_per_s"],
+                progress("%s %-7s B=%-3d waves/CU=%-2d step %3d ms 2^%d keys/lane: %.3e keys/s%s  (%.0f s)" % (
+                    kind, pt["layout"], pt["batch"], pt["waves_per_cu"], pt["step_ms"], pt["lane_keys_log2"],
+                    pt["keys_per_s"],
                     "" if kind == "batched" else ", p50 %.3f s (all %.3f s)" % (pt["p50_s"] or -1,
                                                                               pt["p50_s_with_unfinished"] or -1),
                     time.time() - t0))

@@ -286,6 +286,7 @@ extern "C" int lxv_multi_open(const lxv_multi_opts* opts, lxv_multi** out) {
     m->pipelined = o.pipelined != 0;
     if (o.max_searches) m->max_slots = o.max_searches;
     if (o.max_vlanes) m->max_vlanes = std::max<uint32_t>(64, o.max_vlanes / 64 * 64);
+    if (tuned.lane_keys_log2 >= 8 && tuned.lane_keys_log2 <= 20) m->lane_keys_log2 = tuned.lane_keys_log2;
     if (o.lane_keys_log2) {
         if (o.lane_keys_log2 < 8 || o.lane_keys_log2 > 20) {
             delete m;
