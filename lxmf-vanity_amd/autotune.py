@@ -203,26 +203,28 @@ MIX_SHAPES = ([dict(walk=lxv.WALK_PAIRS, layout=lxv.LAYOUT_8X32, batch=b, waves_
                for b in (64, 128, 256) for w in (8, 16)] +
               [dict(walk=lxv.WALK_PAIRS, layout=lxv.LAYOUT_10X26, batch=256, waves_per_cu=w) for w in (8, 16)])
 SERVING_STEPS_MS = (15, 25, 50)
-# expected keys per virtual lane (log2) for the batched mix: fewer keys per
-# lane means more lanes per search and shorter units of work
+# expected keys per virtual lane (log2): fewer keys per lane means more lanes
+# per search and shorter units of work
 BATCHED_LANE_KEYS_LOG2 = (10, 12, 14)
+SERVING_LANE_KEYS_LOG2 = (8, 10)
 
 
 def mix_sweep(kind, wall_s, progress=print):
-    """Runs the batched mix or the serving stream at every batched shape (and,
-    for serving, every step budget; for the batched mix, every expected keys
-    per virtual lane) for wall_s seconds each."""
+    """Runs the batched mix or the serving stream at every batched shape and
+    every expected keys per virtual lane (and, for serving, every step budget)
+    for wall_s seconds each."""
     import workloads
     points = []
     for sh in MIX_SHAPES:
-        variants = SERVING_STEPS_MS if kind == "serving" else BATCHED_LANE_KEYS_LOG2
-        for v in variants:
+        if kind == "serving":
+            variants = [(step, lk) for step in SERVING_STEPS_MS for lk in SERVING_LANE_KEYS_LOG2]
+        else:
+            variants = [(0, lk) for lk in BATCHED_LANE_KEYS_LOG2]
+        for step_ms, lk in variants:
             shape = {"batch": sh["batch"], "waves_per_cu": sh["waves_per_cu"], "layout": sh["layout"],
-                     "walk": sh["walk"]}
-            step_ms = v if kind == "serving" else 0
+                     "walk": sh["walk"], "lane_keys_log2": lk}
             t0 = time.time()
             if kind == "batched":
-                shape["lane_keys_log2"] = v
                 r = workloads.batched(wall_s=wall_s, shape=shape, progress=None)
                 pt = {"keys_per_s": r["keys_per_s"], "found": r["found"]}
             else:

@@ -60,3 +60,40 @@ def test_mix_pick():
     # serving: lowest p50 among points within 10% of the best rate (8.0e9 is
     # out, 8.5e9 is in)
     assert autotune.mix_pick("serving", pts)["batch"] == 128
+
+
+def test_mix_sweep_runs_every_variant(monkeypatch):
+    # the sweep's bookkeeping without a GPU: each point records the shape,
+    # step budget and keys per lane it was run with, and every combination the
+    # sweep names is run once
+    import workloads
+    calls = []
+
+    def fake(kind):
+        def run(**kw):
+            sh = dict(kw["shape"])
+            calls.append((kind, sh["batch"], sh["waves_per_cu"], sh["layout"], sh.get("step_ms", 0),
+                          sh["lane_keys_log2"]))
+            # a rate that rises with B and keys per lane, a p50 that rises with
+            # the step budget and falls with keys per lane
+            rate = 8e9 + 1e6 * sh["batch"] + 1e7 * sh["lane_keys_log2"]
+            p50 = 0.01 * (sh.get("step_ms") or 1) - 0.001 * sh["lane_keys_log2"]
+            r = {"keys_per_s": rate, "found": 1, "p50_s": p50, "p99_s": 1.0, "p50_s_with_unfinished": p50,
+                 "p99_s_with_unfinished": 2.0, "finished": 1, "unfinished": 0, "policy": "x",
+                 "shape": {"step_ms": sh.get("step_ms") or 200, "lane_keys_log2": sh["lane_keys_log2"]}}
+            return r
+        return run
+
+    monkeypatch.setattr(workloads, "batched", fake("batched"))
+    monkeypatch.setattr(workloads, "serving", fake("serving"))
+    pts = autotune.mix_sweep("batched", 1.0, progress=None)
+    assert len(pts) == len(autotune.MIX_SHAPES) * len(autotune.BATCHED_LANE_KEYS_LOG2)
+    assert sorted({p["lane_keys_log2"] for p in pts}) == sorted(autotune.BATCHED_LANE_KEYS_LOG2)
+    assert autotune.mix_pick("batched", pts)["lane_keys_log2"] == max(autotune.BATCHED_LANE_KEYS_LOG2)
+    pts = autotune.mix_sweep("serving", 1.0, progress=None)
+    n = len(autotune.MIX_SHAPES) * len(autotune.SERVING_STEPS_MS) * len(autotune.SERVING_LANE_KEYS_LOG2)
+    assert len(pts) == n and len(set(calls[-n:])) == n
+    assert {(p["step_ms"], p["lane_keys_log2"]) for p in pts} == \
+        {(s, k) for s in autotune.SERVING_STEPS_MS for k in autotune.SERVING_LANE_KEYS_LOG2}
+    pick = autotune.mix_pick("serving", pts)
+    assert pick["step_ms"] == min(autotune.SERVING_STEPS_MS)
