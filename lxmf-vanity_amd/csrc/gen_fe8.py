@@ -154,17 +154,16 @@ def gen_body(kind):
     g.emit("v_mul_u32_u24 v4, v%d, 38" % t_hi)
     g.emit("v_mad_u64_u32 v[%d:%d], %%[cj], v%d, 38, 0" % (t_lo, t_hi, t_lo))
     g.emit("v_add_u32 v%d, v%d, v4" % (t_hi, t_hi))
-    g.emit("v_add_co_u32_e64 %%[r0], %%[c0], %%[r0], v%d" % t_lo)
-    g.emit("s_nop 1")
-    g.emit("v_addc_co_u32_e64 %%[r1], %%[c1], %%[r1], v%d, %%[c0]" % t_hi)
+    # the chain carries through VCC (VOP2 forms: a carry-in read of VCC
+    # needs no wait states); v_cndmask reads it as a lane mask two later
+    g.emit("v_add_co_u32_e32 %%[r0], vcc, v%d, %%[r0]" % t_lo)
+    g.emit("v_addc_co_u32_e32 %%[r1], vcc, v%d, %%[r1], vcc" % t_hi)
     for i in range(2, 8):
-        g.emit("s_nop 1")
-        g.emit("v_addc_co_u32_e64 %%[r%d], %%[c%d], %%[r%d], 0, %%[c%d]" % (i, i % 2, i, (i - 1) % 2))
+        g.emit("v_addc_co_u32_e32 %%[r%d], vcc, 0, %%[r%d], vcc" % (i, i))
     g.emit("s_nop 1")
-    g.emit("v_cndmask_b32_e64 v4, 0, 38, %[c1]")
-    g.emit("v_add_co_u32_e64 %[r0], %[c0], %[r0], v4")
-    g.emit("s_nop 1")
-    g.emit("v_addc_co_u32_e64 %[r1], %[cj], %[r1], 0, %[c0]")
+    g.emit("v_cndmask_b32_e64 v4, 0, 38, vcc")
+    g.emit("v_add_co_u32_e32 %[r0], vcc, v4, %[r0]")
+    g.emit("v_addc_co_u32_e32 %[r1], vcc, 0, %[r1], vcc")
     return g.ins
 
 
@@ -190,7 +189,7 @@ def emit_function(name, kind):
         lines.append("        \"%s\\n\"" % s)
     lines.append("        : " + ",\n          ".join(outs))
     lines.append("        : " + ",\n          ".join(ins))
-    lines.append("        : \"v0\", \"v1\", \"v2\", \"v3\", \"v4\");")
+    lines.append("        : \"v0\", \"v1\", \"v2\", \"v3\", \"v4\", \"vcc\");")
     lines.append("}")
     return "\n".join(lines), body
 

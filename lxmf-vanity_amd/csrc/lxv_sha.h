@@ -217,9 +217,24 @@ LXV_HD void lxmf_address_words(const uint32_t u[8], const ShaSearch& c, uint32_t
     sha_block3(st, c, out);
 }
 
+// (x & m) ^ w: zero exactly when (x & m) == w (v_bitop3 truth table 0x6c).
+LXV_HD uint32_t masked_diff(uint32_t x, uint32_t w, uint32_t m) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(x, w, m, 0x6c);
+#else
+    return (x & m) ^ w;
+#endif
+}
+
+// (address & mask) == want, word by word. Words 1..3 are tested only when
+// the search's mask or want words there are not all zero (a uniform branch),
+// so a prefix of up to 8 hex digits costs one bitop3 and one compare.
 LXV_HD bool prefix_match(const uint32_t addr[4], const ShaSearch& c) {
-    return ((addr[0] & c.mask[0]) == c.want[0]) & ((addr[1] & c.mask[1]) == c.want[1]) &
-           ((addr[2] & c.mask[2]) == c.want[2]) & ((addr[3] & c.mask[3]) == c.want[3]);
+    uint32_t d = masked_diff(addr[0], c.want[0], c.mask[0]);
+    if (__builtin_expect((c.mask[1] | c.mask[2] | c.mask[3] | c.want[1] | c.want[2] | c.want[3]) != 0, 0))
+        d |= masked_diff(addr[1], c.want[1], c.mask[1]) | masked_diff(addr[2], c.want[2], c.mask[2]) |
+             masked_diff(addr[3], c.want[3], c.mask[3]);
+    return d == 0;
 }
 
 // Host side: the per-search constants from the Ed25519 public key words (big

@@ -319,6 +319,38 @@ def test_full_address_prefix_finds_exactly_that_key(small_backend):
     assert hits == [(t, n, want)]
 
 
+def test_raw_want_mask_words_keep_the_and_compare(small_backend):
+    """LaunchArgs carries raw words and a hit is (address & mask) == want
+    (common.h:344-348): a want bit outside the mask, in the first word or in
+    a word the mask leaves empty, matches nothing; a mask in a later word
+    alone selects by that word, as the oracle's addresses say."""
+    b = small_backend
+    start, _, ed_pub, _ = seeded(55, 0)
+    b.init_seeded(start, STRIDE)
+    b.launch(lxv.launch_args_for(ed_pub, "0"))
+    assert b.stats().hits_total > 1000
+    for word, bit in ((0, 1), (2, 1 << 7)):
+        a = lxv.launch_args_for(ed_pub, "0")
+        a.want[word] |= bit
+        b.init_seeded(start, STRIDE)
+        got, _ = b.launch(a)
+        assert got == [] and b.stats().hits_total == 0, (word, bit)
+    # address byte 8 == 0xa5, nothing else fixed
+    a = lxv.launch_args_for(ed_pub, None)
+    a.want[:] = [0, 0, 0xA5000000, 0]
+    a.mask[:] = [0, 0, 0xFF000000, 0]
+    b.init_seeded(start, STRIDE)
+    got, _ = b.launch(a)
+    assert b.stats().hits_total == len(got)
+    per_lane = b.batch
+    every, _ = orc.scan_ranges(start, ed_pub, [(t * STRIDE, per_lane) for t in range(b.threads)], None,
+                               cap=b.threads * per_lane)
+    assert len(every) == b.threads * per_lane
+    want = [(g, addr) for g, addr in every if addr[8] == 0xA5]
+    assert len(want) > 50
+    assert sorted((t * STRIDE + n, addr) for t, n, addr in got) == want
+
+
 def test_hits_beyond_max_hits_are_counted_not_stored(small_backend):
     b = small_backend
     start, _, ed_pub, _ = seeded(50, 0)
