@@ -97,9 +97,10 @@ struct Pool {
 };
 
 uint32_t vlanes_for(int nibbles, uint32_t cap, uint32_t lane_keys_log2) {
-    // about 16^digits / 2^lane_keys_log2 lanes: at the default 2^14 expected
-    // keys per lane, the ladder that sets a lane up (a few hundred keys' worth
-    // of field work) costs about 1% of the search.
+    // about 16^digits / 2^lane_keys_log2 lanes. A lane is set up by a ladder
+    // over its index (at most 18 bits) for the pair walk, and by a full
+    // scalar ladder for the projective walk (a few hundred keys' worth of
+    // field work: about 1% of a lane's 2^14 expected keys).
     const int log2_lanes = 4 * nibbles - (int)lane_keys_log2;
     uint64_t v = log2_lanes <= 6 ? 64 : (log2_lanes >= 18 ? (1u << 18) : (1ull << log2_lanes));
     if (v > cap) v = cap;
@@ -494,11 +495,6 @@ static void finish(lxv_multi* m, Search& s, int status, uint64_t g, bool found) 
     m->searches.erase(s.id);
 }
 
-// Admits queued searches by (priority, submission order), each as soon as a
-// slot and a pool range of its size are free; one that does not fit yet does
-// not hold back smaller ones behind it. The admitted searches' slot tables
-// are copied from pinned host mirrors and their lanes set up by one ladder
-// launch, all enqueued ahead of the step's walk on the same stream.
 // Whether search r may be suspended to make room: resident, not admitted in
 // this call, not walked by the step still in flight (its progress there is
 // not collected yet), and not the longest-waiting resident search when that
