@@ -18,7 +18,10 @@ carry out of the low half, which the low scan keeps); then the low columns
 computes V = L + 38 H' (L: the low columns with their own carries). Its
 carry-out T < 2^36 + 38 (two words in the accumulator pair) is folded once
 more as 38 T < 2^42, and a final wrap past 2^256 adds 38 (the value is then
-below 2^42, so that addition stops within two words).
+below 2^42, so that addition stops within two words). These two chains carry
+through VCC in the VOP2 forms, whose carry-in read of VCC right after its
+write needs no wait states; only the v_cndmask that reads VCC as a lane mask
+waits two.
 
 The square forms each cross product once against the words of 2A
 (a_i * 2 A_{>i}), so it needs 35 + 8 products instead of 64.
