@@ -314,3 +314,15 @@ def test_cli_cpu_walk_matches_x25519_across_batches():
     for _ in range(3):
         walk += w.next_batch()[1]
     assert [keys[32 * n:32 * n + 32] for n in range(768)] == walk
+
+
+@pytest.mark.parametrize("script,generated", [("gen_fe8.py", "fe8_asm.inc"), ("gen_pair_table.py", "pair_table.inc")])
+def test_generated_sources_match_their_generators(script, generated):
+    # the committed inline asm and point table are exactly what their
+    # generators print, so an edit to either side cannot drift unnoticed
+    import subprocess
+    import sys
+    csrc = os.path.join(ROOT, "lxmf-vanity_amd", "csrc")
+    out = subprocess.run([sys.executable, os.path.join(csrc, script)], capture_output=True, text=True,
+                         check=True, timeout=120).stdout
+    assert out == open(os.path.join(csrc, generated)).read()
