@@ -29,6 +29,9 @@ merged, VALU busy %. The fastest point is the pick; the table goes to JSON.
     # seconds on its workload; picks merged into --out
     python autotune.py --mix batched [--wall-s 20] --out tuned.json
     python autotune.py --mix serving [--wall-s 20] --out tuned.json
+    # a partial sweep (here 10 ms steps at 8 waves/CU) merged into the
+    # earlier one, the pick made over all points
+    python autotune.py --mix serving --steps-ms 10 --waves-per-cu 8 --extend
 
 Mix picks: batched, the highest aggregate keys/s; serving, the lowest p50
 time-to-hit over all requests (unfinished counted at their age) among the
@@ -209,17 +212,22 @@ BATCHED_LANE_KEYS_LOG2 = (10, 12, 14)
 SERVING_LANE_KEYS_LOG2 = (8, 10)
 
 
-def mix_sweep(kind, wall_s, progress=print):
+def mix_sweep(kind, wall_s, progress=print, steps_ms=None, lane_keys_log2=None, waves_per_cu=None):
     """Runs the batched mix or the serving stream at every batched shape and
     every expected keys per virtual lane (and, for serving, every step budget)
-    for wall_s seconds each."""
+    for wall_s seconds each. steps_ms, lane_keys_log2 and waves_per_cu, when
+    given, replace the default lists (a partial sweep, for --extend)."""
     import workloads
     points = []
+    steps = tuple(steps_ms or SERVING_STEPS_MS)
+    lks = tuple(lane_keys_log2 or (SERVING_LANE_KEYS_LOG2 if kind == "serving" else BATCHED_LANE_KEYS_LOG2))
     for sh in MIX_SHAPES:
+        if waves_per_cu and sh["waves_per_cu"] not in waves_per_cu:
+            continue
         if kind == "serving":
-            variants = [(step, lk) for step in SERVING_STEPS_MS for lk in SERVING_LANE_KEYS_LOG2]
+            variants = [(step, lk) for step in steps for lk in lks]
         else:
-            variants = [(0, lk) for lk in BATCHED_LANE_KEYS_LOG2]
+            variants = [(0, lk) for lk in lks]
         for step_ms, lk in variants:
             shape = {"batch": sh["batch"], "waves_per_cu": sh["waves_per_cu"], "layout": sh["layout"],
                      "walk": sh["walk"], "lane_keys_log2": lk}
@@ -244,6 +252,16 @@ def mix_sweep(kind, wall_s, progress=print):
                                                                               pt["p50_s_with_unfinished"] or -1),
                     time.time() - t0))
     return points
+
+
+def merge_points(old, new):
+    """The points of an earlier mix sweep with those of a newer (partial) one;
+    a newer point replaces an older one of the same shape, step budget and
+    keys per lane."""
+    def key(p):
+        return (p["batch"], p["waves_per_cu"], p["layout_id"], p["walk_id"], p["step_ms"], p["lane_keys_log2"])
+    fresh = {key(p) for p in new}
+    return [p for p in old if key(p) not in fresh] + list(new)
 
 
 def mix_pick(kind, points):
@@ -275,6 +293,12 @@ def main():
     ap.add_argument("--mix", choices=("batched", "serving"), default=None,
                     help="sweep the batched kernel's shapes on this workload and merge its pick into --out")
     ap.add_argument("--wall-s", type=float, default=20.0, help="--mix: seconds per point")
+    ap.add_argument("--steps-ms", type=int, nargs="+", default=None, help="--mix serving: step budgets to run")
+    ap.add_argument("--lane-keys-log2", type=int, nargs="+", default=None,
+                    help="--mix: expected keys per virtual lane (log2) to run")
+    ap.add_argument("--waves-per-cu", type=int, nargs="+", default=None, help="--mix: only these waves per CU")
+    ap.add_argument("--extend", action="store_true",
+                    help="--mix: merge the points into the table's earlier sweep and pick over all of them")
     ap.add_argument("--join", nargs="+", default=None,
                     help="tables of separate sweeps (e.g. one per walk form) to join into --out, picking anew")
     ap.add_argument("--walk", choices=("all", "pairs", "projective"), default="all")
@@ -283,7 +307,11 @@ def main():
     args = ap.parse_args()
     if args.mix:
         table = load(args.out) or {"points": []}
-        pts = mix_sweep(args.mix, args.wall_s)
+        pts = mix_sweep(args.mix, args.wall_s, steps_ms=args.steps_ms, lane_keys_log2=args.lane_keys_log2,
+                        waves_per_cu=args.waves_per_cu)
+        earlier = table.get("mix_points", {}).get(args.mix)
+        if args.extend and earlier:
+            pts = merge_points(earlier["points"], pts)
         table.setdefault("mix_points", {})[args.mix] = {
             "workload": ("batched: 256 seeded searches (4-10 hex digits)" if args.mix == "batched" else
                          "serving: Poisson 20/s, 5-9 hex digits, short-first-oldest admission") +

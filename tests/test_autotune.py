@@ -97,3 +97,16 @@ def test_mix_sweep_runs_every_variant(monkeypatch):
         {(s, k) for s in autotune.SERVING_STEPS_MS for k in autotune.SERVING_LANE_KEYS_LOG2}
     pick = autotune.mix_pick("serving", pts)
     assert pick["step_ms"] == min(autotune.SERVING_STEPS_MS)
+    # a partial sweep: 5 ms steps at 8 waves/CU only, merged into the full one;
+    # its points replace nothing else, and the pick is made over all points
+    part = autotune.mix_sweep("serving", 1.0, progress=None, steps_ms=[5], lane_keys_log2=[8], waves_per_cu=[8])
+    shapes8 = [s for s in autotune.MIX_SHAPES if s["waves_per_cu"] == 8]
+    assert len(part) == len(shapes8) and {(p["step_ms"], p["lane_keys_log2"], p["waves_per_cu"]) for p in part} == \
+        {(5, 8, 8)}
+    merged = autotune.merge_points(pts, part)
+    assert len(merged) == len(pts) + len(part)
+    assert autotune.mix_pick("serving", merged)["step_ms"] == 5
+    # a point run again replaces the older one of the same shape and variant
+    again = [dict(p, keys_per_s=1.0) for p in part]
+    assert len(autotune.merge_points(merged, again)) == len(merged)
+    assert all(p["keys_per_s"] == 1.0 for p in autotune.merge_points(merged, again) if p["step_ms"] == 5)
