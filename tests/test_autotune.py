@@ -110,3 +110,14 @@ def test_mix_sweep_runs_every_variant(monkeypatch):
     again = [dict(p, keys_per_s=1.0) for p in part]
     assert len(autotune.merge_points(merged, again)) == len(merged)
     assert all(p["keys_per_s"] == 1.0 for p in autotune.merge_points(merged, again) if p["step_ms"] == 5)
+
+
+def test_committed_picks_follow_from_the_committed_points():
+    # the picks in tuned.json (compiled into the library) are what the pick
+    # rules give on the points recorded beside them
+    table = autotune.load()
+    assert table is not None
+    for kind in ("batched", "serving"):
+        pts = table["mix_points"][kind]["points"]
+        assert autotune.mix_pick(kind, pts) == table["picks"][kind], kind
+    assert autotune.pick({"workload": table["workload"], "points": table["points"]})["pick"] == table["pick"]
